@@ -1,0 +1,160 @@
+"""Benchmark of the constrained-HMC hot path on one MI355X.
+
+Workload (the headline configuration): 32^4 periodic lattice, Iwasaki action
+(c1 = -0.331, beta = 2.6) with 3 levels of isotropic stout smearing
+(rho = 0.1) feeding the force, rectangle force on, constrained Metropolis with
+a demon window of +-0.5% of the initial plaquette sum, links stored FP32 with
+12-real reconstruction, force-gradient integrator with tau = 1 and 12 steps,
+FP64 Hamiltonian / accept. One step = one full trajectory (momentum refresh,
+H_old, 12 FG steps = 48 force evaluations and 49 link updates, H_new, accept).
+Inputs: Haar-random links from a fixed seed (synthetic), resident in HBM.
+
+Prints one JSON line: traj/s (value), link-updates/s, force GB/s, the FP64
+path's traj/s on the same workload, the roofline of the dominant kernel
+measured with HIP events on the library's stream, and the CPU oracle timed on
+a bounded sample of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+M_FLOPS = 198  # one 3x3 complex matrix product
+
+# Algorithmic per-link figures of each kernel family, FP32 storage with
+# 12-real links (bytes: every operand read once and written once; flops:
+# canonical 3x3 products x 198).
+KERNEL_UNITS = {
+    # family: (bytes per link, 3x3 products per link)
+    "stout_fwd": (48 + 32 + 72 + 48, 15),          # read U_k; write Q_k, C_k, U_k+1
+    "stout_top": (48 + 48 + 32 + 72 + 72 + 72, 85 + 6),   # Iwasaki staples of U_n + Lambda of level n-1
+    "stout_bwd": (48 + 72 + 72 + 48 + 32 + 72 + 72 + 72, 54 + 6),  # six-term chain + next Lambda
+    "stout_bwd_force": (48 + 72 + 72 + 32 + 32, 54 + 1),  # six-term chain at level 0 + P += eps F
+    "link_update": (48 + 32 + 48, 3),
+    "force_iwasaki": (48 + 32 + 32, 85),
+}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--L", type=int, default=32)
+    p.add_argument("--n-md", type=int, default=12)
+    p.add_argument("--n-stout", type=int, default=3)
+    p.add_argument("--fp64-steps", type=int, default=3, help="trajectories of the FP64 comparison (0: skip)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--seed", type=int, default=32)
+    return p.parse_args()
+
+
+def cpu_baseline(L_full, n_md, n_stout):
+    """The FP64 CPU oracle on a bounded sample: one force evaluation, one link
+    update and one Hamiltonian of the same action on a 16^4 lattice, scaled by
+    volume to a 32^4 trajectory with the FG schedule (4 n forces, 4 n + 1 link
+    updates, 2 Hamiltonians)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    Ls = (16, 16, 16, 16)
+    a = O.action(1, 2.6, -0.331, n_stout=n_stout, rho=0.1)
+    U, P = O.haar_links(Ls, 7), O.init_momentum(Ls, 7, 0)
+    t0 = time.perf_counter()
+    O.get_force(Ls, U, a)
+    t1 = time.perf_counter()
+    O.evolve_gauge_field(Ls, U, P, 0.01, 0)
+    t2 = time.perf_counter()
+    O.hamiltonian(Ls, U, P, a)
+    t3 = time.perf_counter()
+    scale = (L_full ** 4) / (16 ** 4)
+    t_traj = scale * (4 * n_md * (t1 - t0) + (4 * n_md + 1) * (t2 - t1) + 2 * (t3 - t2))
+    return {"value": 1.0 / t_traj, "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"FP64 oracle on 16^4: 1 force ({t1 - t0:.2f} s), 1 link update ({t2 - t1:.2f} s), "
+                      f"1 Hamiltonian ({t3 - t2:.2f} s); scaled x{scale:.0f} by volume to one {L_full}^4 "
+                      f"trajectory ({4 * n_md} forces, {4 * n_md + 1} updates, 2 H) = {t_traj:.1f} s"}
+
+
+def run_chain(M, ctx, L, arg, seed, precision, steps, warmup, timing):
+    ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005)
+    ch.haar(seed)
+    ch.run(warmup)
+    ctx.synchronize()
+    if timing:
+        ctx.reset_timing()
+        ctx.set_timing(True)
+    t0 = time.perf_counter()
+    res = ch.run(steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.set_timing(False)
+    return dt, res
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)", file=sys.stderr)
+        sys.exit(2)
+    import mag_amd as M
+    ctx = M.Context(0)
+    L, n = a.L, a.n_md
+    arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=2.6, gauge=M.Gauge(M.IWASAKI, -0.331), n_stout=a.n_stout,
+                     rho=0.1, num_forced_accept_step=a.warmup)
+    V = L ** 4
+    nlinks = 4 * V
+    print(f"bench: {L}^4 Iwasaki+{a.n_stout}stout FP32-MD, warmup {a.warmup}, steps {a.steps}", file=sys.stderr,
+          flush=True)
+    dt, res = run_chain(M, ctx, L, arg, a.seed, M.F32, a.steps, a.warmup, True)
+    fams = {}
+    for f in list(KERNEL_UNITS) + ["stout_fwd_h", "action", "kinetic", "reduce", "mom_refresh", "reunitarize"]:
+        ms, nl = ctx.kernel_time(f)
+        if nl:
+            fams[f] = (ms, nl)
+    traj_s = a.steps / dt
+    nforce = 4 * n * a.steps
+    force_ms = sum(fams.get(f, (0, 0))[0] for f in ("stout_fwd", "stout_top", "stout_bwd", "stout_bwd_force",
+                                                     "force_iwasaki"))
+    force_bytes_link = sum(KERNEL_UNITS[f][0] * (a.n_stout if f == "stout_fwd" else
+                                                 (a.n_stout - 1 if f == "stout_bwd" else 1))
+                           for f in ("stout_fwd", "stout_top", "stout_bwd", "stout_bwd_force"))
+    force_gbs = force_bytes_link * nlinks * nforce / (force_ms * 1e-3) / 1e9 if force_ms else None
+    dom = max((f for f in fams if f in KERNEL_UNITS), key=lambda f: fams[f][0])
+    ms, nl = fams[dom]
+    avg_s = ms * 1e-3 / nl
+    b_link, m_link = KERNEL_UNITS[dom]
+    gbs = b_link * nlinks / avg_s / 1e9
+    tflops = m_link * M_FLOPS * nlinks / avg_s / 1e12
+    roof = {"bound": "valu", "kernel": dom, "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "hbm_achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "avg_launch_ms": round(avg_s * 1e3, 4)}
+    out = {"metric": "traj/s + force GB/s at 32^4 Iwasaki+3stout, FP32-MD vs FP64; chain p99 latency",
+           "value": round(traj_s, 4), "unit": "traj/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (Haar-random links, seed %d)" % a.seed,
+           "config": {"workload": f"C3: {L}^4 Iwasaki(c1=-0.331, beta=2.6) + {a.n_stout}-stout(rho=0.1), "
+                                  f"FG tau=1 n={n}, FP32 12-real links, FP64 dH, demon window 0.5%",
+                      "lattice": [L] * 4, "parallelism": "single GPU"},
+           "link_updates_per_s": round(4 * V * n * traj_s, 1),
+           "force_gbs": round(force_gbs, 1) if force_gbs else None,
+           "accept_rate": round(sum(r.accept for r in res) / len(res), 3),
+           "kernel_ms_per_traj": {f: round(v[0] / a.steps, 3) for f, v in fams.items()},
+           "roofline": roof}
+    if a.fp64_steps > 0:
+        dt64, r64 = run_chain(M, ctx, L, arg, a.seed, M.F64, a.fp64_steps, 1, False)
+        out["fp64_traj_per_s"] = round(a.fp64_steps / dt64, 4)
+        out["fp32_over_fp64"] = round(traj_s / (a.fp64_steps / dt64), 3)
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(L, n, a.n_stout)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

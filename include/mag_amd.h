@@ -1,0 +1,187 @@
+/*
+ * mag_amd — MI355X-native constrained hybrid Monte Carlo for SU(3) lattice
+ * gauge fields: the C ABI of libmag_amd.so.
+ *
+ * The reference (hummingtree/mag) exposes its hot path as namespace-level
+ * inline C++ over qlat::Field<cps::Matrix>; it has no FFI. This ABI is the
+ * boundary a drop-in needs: every entry point below names the reference
+ * function it replaces (file:line). Conventions:
+ *   - handle based; plain pointers and sizes; no C++ or torch types;
+ *   - every call returns 0 (MAG_OK) or a negative status; mag_last_error()
+ *     gives the message of the last failure on a context;
+ *   - fields are allocated by the caller through mag_field_create and live on
+ *     the context's device; kernels never allocate on the hot path (the
+ *     context owns all workspace, created on first use for a lattice shape);
+ *   - host interop uses the reference's qlat layout: AoS doubles
+ *     [site][mu][3][3][re,im], site = x0 + L0 (x1 + L1 (x2 + L2 x3));
+ *     algebra fields (momenta, forces) are passed as Hermitian traceless 3x3
+ *     matrices, as the reference stores them;
+ *   - calls returning an FP64 scalar synchronise the context's stream.
+ * One context per host thread at a time; contexts share nothing.
+ */
+#ifndef MAG_AMD_H
+#define MAG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mag_ctx_s* mag_ctx;
+typedef struct mag_field_s* mag_field;
+typedef struct mag_chain_s* mag_chain;
+typedef struct mag_server_s* mag_server;
+
+/* status codes */
+enum {
+  MAG_OK = 0,
+  MAG_ERR_ARG = -1,         /* bad argument or mismatched fields */
+  MAG_ERR_GEOMETRY = -2,    /* lattice extents not supported */
+  MAG_ERR_OOM = -3,         /* device allocation failed */
+  MAG_ERR_DEVICE = -4,      /* HIP runtime error */
+  MAG_ERR_NAN = -5,         /* non-finite Hamiltonian: trajectory rejected */
+  MAG_ERR_UNSUPPORTED = -6, /* combination not implemented */
+  MAG_ERR_BUDGET = -7,      /* chain does not fit the HBM budget */
+  MAG_ERR_STATE = -8        /* call not valid in the object's current state */
+};
+
+/* field kinds and precisions */
+enum { MAG_LINK18 = 0, MAG_LINK12 = 1, MAG_ALG8 = 2 };
+enum { MAG_F32 = 0, MAG_F64 = 1 };
+/* gauge actions: Wilson, or the Iwasaki form with c0 = 1 - 8 c1 (Iwasaki
+ * c1 = -0.331, DBW2 c1 = -1.4069); Gauge{type, c1}, field-hmc.h:344, 384 */
+enum { MAG_WILSON = 0, MAG_IWASAKI = 1 };
+
+typedef struct {
+  int type;      /* MAG_WILSON or MAG_IWASAKI */
+  double beta;
+  double c1;     /* rectangle coefficient (Iwasaki form) */
+  int mag;       /* constraint block size, 0 = unconstrained (Arg_chmc::mag) */
+  int n_stout;   /* isotropic stout levels feeding the action (0 = thin) */
+  double rho;    /* stout weight */
+} mag_action;
+
+/* Arg_chmc (field-hmc.h:294-307) plus the build's run controls. */
+typedef struct {
+  int trajectory_length;   /* MD steps per trajectory */
+  double dt;               /* step size */
+  int integrator;          /* 0 force gradient (field-hmc.h:525), 1 leapfrog (:569) */
+  int num_forced_accept;   /* num_forced_accept_step */
+  int accept_mode;         /* 0 Metropolis, 1 demon window, 2 both */
+  double demon_emax;       /* > 0: window half-width; < 0: fraction of |S0| */
+  uint64_t seed;           /* Philox key of the chain */
+  int reunitarize;         /* 1: reunitarize after every trajectory */
+  int precision;           /* MAG_F64: all FP64; MAG_F32: FP32 MD, FP64 H */
+  int recon12;             /* FP32 links stored as 12 reals */
+} mag_hmc_params;
+
+typedef struct {
+  double h_old, h_new, dh, die_roll;
+  int accept;
+  double avg_plaq;         /* of the chain state after the accept step */
+  double s_gauge;          /* gauge action of that state */
+  double demon_e[2];       /* demon energies (plaquette, rectangle sums) */
+  double reunit_dev;       /* max deviation from reunitarize */
+  int status;              /* MAG_OK or MAG_ERR_NAN */
+} mag_traj_result;
+
+/* ---------------------------------------------------------------- context */
+int mag_ctx_create(int device, mag_ctx* out);
+int mag_ctx_destroy(mag_ctx ctx);
+const char* mag_last_error(mag_ctx ctx);
+int mag_ctx_synchronize(mag_ctx ctx);
+/* Total device bytes held by a context's fields and workspace. */
+int mag_ctx_memory(mag_ctx ctx, size_t* bytes);
+/* Time of the kernels of the last timed region (see mag_ctx_set_timing):
+ * per-kernel-family event time, milliseconds. */
+int mag_ctx_set_timing(mag_ctx ctx, int on);
+int mag_ctx_kernel_time(mag_ctx ctx, const char* family, double* ms, long* launches);
+int mag_ctx_reset_timing(mag_ctx ctx);
+
+/* ---------------------------------------------------------------- fields
+ * qlat::Field<cps::Matrix>::init(geo) with multiplicity 4 (field-hmc.h:777-781) */
+int mag_field_create(mag_ctx ctx, const int dims[4], int kind, int prec, mag_field* out);
+int mag_field_destroy(mag_field f);
+int mag_field_upload_aos(mag_field f, const double* host);
+int mag_field_download_aos(mag_field f, double* host);
+int mag_field_copy(mag_field dst, mag_field src); /* gFieldExt = gField (field-hmc.h:829) */
+int mag_field_info(mag_field f, int dims[4], int* kind, int* prec, size_t* bytes);
+
+/* ------------------------------------------------------------ link algebra */
+int mag_unit_links(mag_field U);                   /* UnitMatrix start (quench_test/main.C:92-97) */
+int mag_haar_links(mag_field U, uint64_t seed);    /* hot start; Philox Haar SU(3) */
+/* init_momentum (field-hmc.h:646-693): P = sum_a T_a g_a, g ~ N(0, sqrt m) */
+int mag_init_momentum(mag_field P, uint64_t seed, uint64_t traj, double m);
+/* reunitarize (field-matrix.h:83-100) */
+int mag_reunitarize(mag_field U, double* max_dev);
+/* expiQ (field-hmc.h:82-123), batch of n 3x3 matrices [n][18]; prec selects
+ * the arithmetic of the 3x3 algebra */
+int mag_expiQ_batch(mag_ctx ctx, const double* Q, double* out, long n, int prec);
+/* compute_Lambda (field-multi.h:180-269), batch of n */
+int mag_compute_Lambda_batch(mag_ctx ctx, const double* Q, const double* SigmaP, const double* U, double* out,
+                             long n, int prec);
+
+/* ------------------------------------------------------- forces and updates */
+/* get_force (field-hmc.h:339-425); with n_stout > 0 the force of the smeared
+ * action, by the stout chain of md::get_Fforce (field-multi-md.h:116-368) in
+ * its isotropic form. F is MAG_ALG8 of U's precision. */
+int mag_gauge_force(mag_field F, mag_field U, const mag_action* act);
+/* evolve_gauge_field (field-hmc.h:442-482) */
+int mag_evolve_gauge(mag_field U, mag_field P, double eps, int mag);
+/* evolve_momentum (field-hmc.h:427-440) */
+int mag_evolve_momentum(mag_field P, mag_field F, double eps, double m);
+/* isotropic stout smearing, n levels; out has U's kind and precision */
+int mag_stout_smear(mag_field out, mag_field U, double rho, int nlev);
+/* force_gradient_integrator (field-hmc.h:525-567; md:: variant
+ * field-multi-md.h:599-661); reunitarizes at the end when reunit != 0 */
+int mag_fg_integrate(mag_field U, mag_field P, const mag_action* act, int steps, double dt, int reunit,
+                     double* max_dev);
+/* leap_frog_integrator (field-hmc.h:569-587) */
+int mag_leapfrog_integrate(mag_field U, mag_field P, const mag_action* act, int steps, double dt);
+
+/* -------------------------------------------------------------- reductions
+ * All accumulate in FP64 on the device, in a fixed order. */
+int mag_total_plaq(mag_field U, double* out);          /* field-matrix.h:299-330 */
+int mag_total_rectangular(mag_field U, double* out);   /* field-matrix.h:354-385 */
+int mag_avg_plaquette(mag_field U, double* out);       /* field-matrix.h:387-389 */
+/* potential part of get_hamiltonian (field-hmc.h:632-640) on the smeared field */
+int mag_gauge_action(mag_field U, const mag_action* act, double* S);
+int mag_kinetic_energy(mag_field P, int mag, double* K);
+/* get_hamiltonian (field-hmc.h:589-644): parts = {kinetic, potential} */
+int mag_hamiltonian(mag_field U, mag_field P, const mag_action* act, double parts[2], double* H);
+/* avg_wilson_loop (field-wloop.h:58-98) */
+int mag_avg_wilson_loop(mag_field U, const int r[3], int T, double* out);
+
+/* -------------------------------------------------------------- HMC chain
+ * run_chmc / update_field (field-hmc.h:770-986) as a device-resident chain:
+ * links, backup, momenta, workspace and RNG state stay in HBM. */
+int mag_chain_create(mag_ctx ctx, const int dims[4], const mag_action* act, const mag_hmc_params* p,
+                     mag_chain* out);
+int mag_chain_destroy(mag_chain c);
+int mag_chain_set_links_aos(mag_chain c, const double* host);
+int mag_chain_get_links_aos(mag_chain c, double* host);
+int mag_chain_haar_links(mag_chain c, uint64_t seed);
+int mag_chain_links(mag_chain c, mag_field* U);  /* borrowed view of the chain's links */
+/* Run ntraj trajectories; results[i] for each (may be NULL). Trajectory
+ * indices continue across calls (they key the momentum and die-roll streams). */
+int mag_chain_run(mag_chain c, int ntraj, mag_traj_result* results);
+/* FP64 reversibility check from the current state: trajectory forward,
+ * momenta flipped, trajectory back. Returns max |U_fwd->bwd - U| and the
+ * Hamiltonian change of the round trip; the chain state is unchanged. */
+int mag_chain_reversibility(mag_chain c, double* max_link_dev, double* dh_roundtrip);
+int mag_chain_demon_state(mag_chain c, double state[4]); /* S0p, S0r, Ep, Er */
+int mag_chain_traj_index(mag_chain c, long* next_traj);
+
+/* -------------------------------------------------------------- demon
+ * demon_microcanonical (field-demon.h:25-126) in the reference's sequential
+ * order (sites in index order, mu, 2 hits), executed by one wavefront on the
+ * device; E[4] in/out (plaquette, rectangle, chair, twist). */
+int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
+                             long acc_rej[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAG_AMD_H */

@@ -1,0 +1,349 @@
+"""mag_amd — MI355X-native constrained HMC for SU(3) lattice gauge fields.
+
+Python mirror of the reference's field/HMC entry points (hummingtree/mag:
+field-hmc.h, field-matrix.h, field-multi.h, field-wloop.h, field-demon.h),
+calling the gfx950 kernels of libmag_amd.so through its C ABI. Device fields
+are handles; host arrays use the reference's qlat layout, float64 of shape
+(V, 4, 18) = [site][mu][3x3 complex, interleaved re/im].
+"""
+import ctypes as C
+from dataclasses import dataclass, field as dfield
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ALG8, F32, F64, IWASAKI, LINK12, LINK18, WILSON, Action, HmcParams, MagError,
+                   TrajResult)
+
+__all__ = [
+    "Context", "Field", "Gauge", "Arg_chmc", "Chain", "MagError",
+    "get_force", "evolve_gauge_field", "evolve_momentum", "force_gradient_integrator", "leap_frog_integrator",
+    "get_hamiltonian", "total_plaq", "total_rectangular", "avg_plaquette", "gauge_action", "kinetic_energy",
+    "reunitarize", "init_momentum", "expiQ", "compute_Lambda", "stout_smear", "avg_wilson_loop",
+    "demon_microcanonical", "run_chmc",
+]
+
+_D = C.POINTER(C.c_double)
+
+
+def _check(code, ctx=None):
+    if code != 0:
+        msg = ""
+        if ctx is not None:
+            m = _lib.lib().mag_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise MagError(code, msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+class Context:
+    """A device context (mag_ctx): one HIP stream plus the workspace it owns."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        _check(_lib.lib().mag_ctx_create(device, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _lib.lib().mag_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(_lib.lib().mag_ctx_synchronize(self._h), self._h)
+
+    def memory(self):
+        b = C.c_size_t()
+        _check(_lib.lib().mag_ctx_memory(self._h, C.byref(b)), self._h)
+        return b.value
+
+    def set_timing(self, on):
+        _check(_lib.lib().mag_ctx_set_timing(self._h, 1 if on else 0), self._h)
+
+    def reset_timing(self):
+        _check(_lib.lib().mag_ctx_reset_timing(self._h), self._h)
+
+    def kernel_time(self, family):
+        ms, n = C.c_double(), C.c_long()
+        _check(_lib.lib().mag_ctx_kernel_time(self._h, family.encode(), C.byref(ms), C.byref(n)), self._h)
+        return ms.value, n.value
+
+
+class Field:
+    """Device field (qlat::Field<cps::Matrix> of multiplicity 4)."""
+
+    def __init__(self, ctx, dims, kind=LINK18, prec=F64, _handle=None):
+        self.ctx = ctx
+        self.dims = tuple(int(d) for d in dims)
+        self.kind, self.prec = kind, prec
+        if _handle is not None:
+            self._h, self._owned = _handle, False
+        else:
+            self._h, self._owned = C.c_void_p(), True
+            _check(_lib.lib().mag_field_create(ctx.handle, (C.c_int * 4)(*self.dims), kind, prec, C.byref(self._h)),
+                   ctx.handle)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def volume(self):
+        return int(np.prod(self.dims))
+
+    def __del__(self):
+        try:
+            if self._owned and self._h:
+                _lib.lib().mag_field_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64).reshape(self.volume, 4, 18)
+        _check(_lib.lib().mag_field_upload_aos(self._h, _dptr(a)), self.ctx.handle)
+        return self
+
+    def download(self):
+        a = np.empty((self.volume, 4, 18))
+        _check(_lib.lib().mag_field_download_aos(self._h, _dptr(a)), self.ctx.handle)
+        return a
+
+    def copy_from(self, other):
+        _check(_lib.lib().mag_field_copy(self._h, other.handle), self.ctx.handle)
+        return self
+
+    def unit(self):
+        _check(_lib.lib().mag_unit_links(self._h), self.ctx.handle)
+        return self
+
+    def haar(self, seed):
+        _check(_lib.lib().mag_haar_links(self._h, seed), self.ctx.handle)
+        return self
+
+
+# ----------------------------------------------------------------- arguments
+@dataclass
+class Gauge:
+    """Gauge{type, c1} (field-hmc.h:344, 384): WILSON or IWASAKI form (c0 = 1 - 8 c1)."""
+    type: int = WILSON
+    c1: float = 0.0
+
+
+@dataclass
+class Arg_chmc:
+    """Arg_chmc (field-hmc.h:294-307) plus smearing and run controls."""
+    mag: int = 0
+    trajectory_length: int = 8
+    num_trajectory: int = 1
+    num_step_between_output: int = 1
+    num_forced_accept_step: int = 0
+    num_step_before_output: int = 0
+    beta: float = 6.0
+    dt: float = 0.125
+    gauge: Gauge = dfield(default_factory=Gauge)
+    n_stout: int = 0
+    rho: float = 0.1
+
+    def action(self):
+        return Action(self.gauge.type, self.beta, self.gauge.c1, self.mag, self.n_stout, self.rho)
+
+
+def _act(arg):
+    return arg if isinstance(arg, Action) else arg.action()
+
+
+# ----------------------------------------------------------------- functions
+def get_force(F, U, arg):
+    """get_force (field-hmc.h:339-425); smeared force when arg.n_stout > 0."""
+    a = _act(arg)
+    _check(_lib.lib().mag_gauge_force(F.handle, U.handle, C.byref(a)), U.ctx.handle)
+
+
+def evolve_gauge_field(U, P, dt, arg):
+    """evolve_gauge_field (field-hmc.h:442-482)."""
+    _check(_lib.lib().mag_evolve_gauge(U.handle, P.handle, dt, _act(arg).mag), U.ctx.handle)
+
+
+def evolve_momentum(P, F, dt, arg=None, m=1.0):
+    """evolve_momentum (field-hmc.h:427-440)."""
+    _check(_lib.lib().mag_evolve_momentum(P.handle, F.handle, dt, m), P.ctx.handle)
+
+
+def force_gradient_integrator(U, P, arg, reunit=True):
+    """force_gradient_integrator (field-hmc.h:525-567); returns the reunitarize deviation."""
+    a = _act(arg)
+    d = C.c_double()
+    _check(_lib.lib().mag_fg_integrate(U.handle, P.handle, C.byref(a), arg.trajectory_length, arg.dt,
+                                       1 if reunit else 0, C.byref(d)), U.ctx.handle)
+    return d.value
+
+
+def leap_frog_integrator(U, P, arg):
+    """leap_frog_integrator (field-hmc.h:569-587)."""
+    a = _act(arg)
+    _check(_lib.lib().mag_leapfrog_integrate(U.handle, P.handle, C.byref(a), arg.trajectory_length, arg.dt),
+           U.ctx.handle)
+
+
+def get_hamiltonian(U, P, arg):
+    """get_hamiltonian (field-hmc.h:589-644): returns (H, [kinetic, potential])."""
+    a = _act(arg)
+    parts = (C.c_double * 2)()
+    h = C.c_double()
+    _check(_lib.lib().mag_hamiltonian(U.handle, P.handle, C.byref(a), parts, C.byref(h)), U.ctx.handle)
+    return h.value, [parts[0], parts[1]]
+
+
+def _scalar(fn, *args, ctx):
+    v = C.c_double()
+    _check(fn(*args, C.byref(v)), ctx)
+    return v.value
+
+
+def total_plaq(U):
+    """total_plaq (field-matrix.h:299-330)."""
+    return _scalar(_lib.lib().mag_total_plaq, U.handle, ctx=U.ctx.handle)
+
+
+def total_rectangular(U):
+    """total_rectangular (field-matrix.h:354-385)."""
+    return _scalar(_lib.lib().mag_total_rectangular, U.handle, ctx=U.ctx.handle)
+
+
+def avg_plaquette(U):
+    """avg_plaquette (field-matrix.h:387-389)."""
+    return _scalar(_lib.lib().mag_avg_plaquette, U.handle, ctx=U.ctx.handle)
+
+
+def gauge_action(U, arg):
+    a = _act(arg)
+    return _scalar(_lib.lib().mag_gauge_action, U.handle, C.byref(a), ctx=U.ctx.handle)
+
+
+def kinetic_energy(P, mag=0):
+    return _scalar(_lib.lib().mag_kinetic_energy, P.handle, mag, ctx=P.ctx.handle)
+
+
+def reunitarize(U):
+    """reunitarize (field-matrix.h:83-100): returns the max deviation."""
+    return _scalar(_lib.lib().mag_reunitarize, U.handle, ctx=U.ctx.handle)
+
+
+def init_momentum(P, seed, traj=0, m=1.0):
+    """init_momentum (field-hmc.h:646-693) from the Philox momentum stream."""
+    _check(_lib.lib().mag_init_momentum(P.handle, seed, traj, m), P.ctx.handle)
+
+
+def stout_smear(out, U, rho, nlev):
+    _check(_lib.lib().mag_stout_smear(out.handle, U.handle, rho, nlev), U.ctx.handle)
+
+
+def avg_wilson_loop(U, r, T):
+    """avg_wilson_loop (field-wloop.h:58-98)."""
+    return _scalar(_lib.lib().mag_avg_wilson_loop, U.handle, (C.c_int * 3)(*r), T, ctx=U.ctx.handle)
+
+
+def _batch(ctx, fn, mats, prec):
+    arrs = [np.ascontiguousarray(np.asarray(m, dtype=np.complex128)).view(np.float64).reshape(-1, 18) for m in mats]
+    n = arrs[0].shape[0]
+    out = np.empty((n, 18))
+    _check(fn(ctx.handle, *[_dptr(a) for a in arrs], _dptr(out), n, prec), ctx.handle)
+    return out.view(np.complex128).reshape(np.asarray(mats[0]).shape)
+
+
+def expiQ(ctx, Q, prec=F64):
+    """expiQ (field-hmc.h:82-123) on the device for a batch (..., 3, 3)."""
+    return _batch(ctx, _lib.lib().mag_expiQ_batch, [Q], prec)
+
+
+def compute_Lambda(ctx, Q, SigmaP, U, prec=F64):
+    """compute_Lambda (field-multi.h:180-269) on the device for a batch."""
+    return _batch(ctx, _lib.lib().mag_compute_Lambda_batch, [Q, SigmaP, U], prec)
+
+
+def demon_microcanonical(U, nsweeps, E_max, seed, E=None):
+    """demon_microcanonical (field-demon.h:25-126): returns (E[4], (accept, reject))."""
+    e = np.zeros(4) if E is None else np.array(E, dtype=np.float64)
+    ar = (C.c_long * 2)()
+    _check(_lib.lib().mag_demon_microcanonical(U.handle, nsweeps, E_max, seed, _dptr(e), ar), U.ctx.handle)
+    return e, (ar[0], ar[1])
+
+
+# ----------------------------------------------------------------- chain
+class Chain:
+    """Device-resident HMC chain (run_chmc / update_field, field-hmc.h:770-986)."""
+
+    def __init__(self, ctx, dims, arg, seed=1234, precision=F64, integrator=0, accept_mode=0, demon_emax=0.0,
+                 reunit=True):
+        self.ctx = ctx
+        self.dims = tuple(dims)
+        self.arg = arg
+        self.act = _act(arg)
+        self.params = HmcParams(arg.trajectory_length, arg.dt, integrator, arg.num_forced_accept_step, accept_mode,
+                                demon_emax, seed, 1 if reunit else 0, precision, 1 if precision == F32 else 0)
+        self._h = C.c_void_p()
+        _check(_lib.lib().mag_chain_create(ctx.handle, (C.c_int * 4)(*self.dims), C.byref(self.act),
+                                           C.byref(self.params), C.byref(self._h)), ctx.handle)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().mag_chain_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def set_links(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        _check(_lib.lib().mag_chain_set_links_aos(self._h, _dptr(a)), self.ctx.handle)
+
+    def get_links(self):
+        a = np.empty((int(np.prod(self.dims)), 4, 18))
+        _check(_lib.lib().mag_chain_get_links_aos(self._h, _dptr(a)), self.ctx.handle)
+        return a
+
+    def haar(self, seed):
+        _check(_lib.lib().mag_chain_haar_links(self._h, seed), self.ctx.handle)
+
+    def links(self):
+        h = C.c_void_p()
+        _check(_lib.lib().mag_chain_links(self._h, C.byref(h)), self.ctx.handle)
+        kind = LINK12 if self.params.precision == F32 else LINK18
+        return Field(self.ctx, self.dims, kind, self.params.precision, _handle=h)
+
+    def run(self, ntraj):
+        res = (TrajResult * max(ntraj, 1))()
+        _check(_lib.lib().mag_chain_run(self._h, ntraj, res), self.ctx.handle)
+        return [res[i] for i in range(ntraj)]
+
+    def reversibility(self):
+        d, dh = C.c_double(), C.c_double()
+        _check(_lib.lib().mag_chain_reversibility(self._h, C.byref(d), C.byref(dh)), self.ctx.handle)
+        return d.value, dh.value
+
+    def demon_state(self):
+        s = np.zeros(4)
+        _check(_lib.lib().mag_chain_demon_state(self._h, _dptr(s)), self.ctx.handle)
+        return s
+
+
+def run_chmc(ctx, U_host, arg, seed=1234, **kw):
+    """run_chmc (field-hmc.h:770-904) without file output: returns (final links, per-trajectory results)."""
+    dims = kw.pop("dims")
+    ch = Chain(ctx, dims, arg, seed=seed, **kw)
+    ch.set_links(U_host)
+    res = ch.run(arg.num_trajectory)
+    return ch.get_links(), res

@@ -1,0 +1,243 @@
+// demon_microcanonical (field-demon.h:25-126) on the device, in the
+// reference's sequential order: links in site-index order, mu inner, two hits
+// per link, each accepted hit changing the link before the next link's
+// staples are formed. The chain is inherently serial (one demon energy, one
+// RNG stream), so one wavefront runs it: the 64 lanes split the 168 staple
+// paths of a link (6 plaquette, 18 rectangle, 72 chair, 72 twist; index
+// tables of field-staple.h:27-135 with their defects kept), the four sums are
+// reduced in a fixed lane order through LDS, and lane 0 makes the proposals
+// and the window test. One launch per sweep.
+#include <cstring>
+#include <vector>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using namespace mag;
+
+namespace {
+
+constexpr int NPATH = 168;  // per direction
+constexpr int PLEN = 5;
+
+struct DemonTable {
+  // dirs[mu][path][k]: 0..3 forward, 4..7 backward, -1 = end of path
+  signed char dirs[4][NPATH][PLEN];
+  unsigned char kind[NPATH];  // 0 plaquette, 1 rectangle, 2 chair, 3 twist
+};
+
+// Paths of get_staple_dagger (field-hmc.h:226-240), get_rectangular_dagger
+// (field-hmc.h:137-201), init_chair_index / init_twist_index
+// (field-staple.h:27-135).
+void build_table(DemonTable& t) {
+  std::memset(t.dirs, -1, sizeof t.dirs);
+  const int D = 4;
+  for (int mu = 0; mu < 4; ++mu) {
+    int n = 0;
+    auto put = [&](std::initializer_list<int> d, int kind) {
+      int k = 0;
+      for (int v : d) t.dirs[mu][n][k++] = (signed char)v;
+      t.kind[n] = (unsigned char)kind;
+      ++n;
+    };
+    for (int nu = 0; nu < D; ++nu) {
+      if (nu == mu) continue;
+      put({nu, mu, nu + D}, 0);
+      put({nu + D, mu, nu}, 0);
+    }
+    for (int nu = 0; nu < D; ++nu) {
+      if (nu == mu) continue;
+      put({nu, mu, mu, nu + D, mu + D}, 1);
+      put({nu + D, mu, mu, nu, mu + D}, 1);
+      put({nu, nu, mu, nu + D, nu + D}, 1);
+      put({nu + D, nu + D, mu, nu, nu}, 1);
+      put({mu + D, nu, mu, mu, nu + D}, 1);
+      put({mu + D, nu + D, mu, mu, nu}, 1);
+    }
+    for (int twist = 0; twist < 2; ++twist)
+      for (int nu = 0; nu < D; ++nu) {
+        if (nu == mu) continue;
+        for (int la = 0; la < D; ++la) {
+          if (la == nu || la == mu) continue;
+          int b[4][4];
+          if (!twist) {
+            int c[4][4] = {{nu, la, la + D, nu + D}, {nu, la + D, la + D, nu},
+                           {nu + D, la, la, nu + D}, {nu + D, la + D, la, nu}};
+            std::memcpy(b, c, sizeof b);
+          } else {
+            int c[4][4] = {{nu, la, nu + D, la + D}, {nu, la + D, nu + D, la},
+                           {nu + D, la, nu, la + D}, {nu + D, la + D, nu, la}};
+            std::memcpy(b, c, sizeof b);
+          }
+          for (int k = 0; k < 4; ++k)
+            for (int i = 1; i < 4; ++i) {
+              int d[5], m = 0;
+              for (int j = 0; j < 4; ++j) {
+                if (j == i) d[m++] = mu;
+                d[m++] = b[k][j];
+              }
+              t.dirs[mu][n][0] = (signed char)d[0];
+              for (int j = 1; j < 5; ++j) t.dirs[mu][n][j] = (signed char)d[j];
+              t.kind[n] = (unsigned char)(2 + twist);
+              ++n;
+            }
+        }
+      }
+  }
+}
+
+struct DemonState {
+  double E[4];
+  long long acc, rej;
+};
+
+__global__ __launch_bounds__(64) void k_demon_sweep(LinkView<double, 18> U, Geom g, const DemonTable* tab,
+                                                    DemonState* st, double emax, uint64_t seed, int sweep) {
+  __shared__ double part[72][64];
+  __shared__ double tot[72];
+  const int lane = threadIdx.x;
+  double E[4] = {st->E[0], st->E[1], st->E[2], st->E[3]};
+  long long acc = st->acc, rej = st->rej;
+  for (int s = 0; s < g.V; ++s) {
+    const Site p0 = site_of(g, s);
+    for (int mu = 0; mu < 4; ++mu) {
+      M3<double> sum[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sum[k] = zero3<double>();
+      for (int q = lane; q < NPATH; q += 64) {
+        Site y = p0;
+        M3<double> m = unit3<double>();
+        for (int j = 0; j < PLEN; ++j) {
+          const int d = tab->dirs[mu][q][j];
+          if (d < 0) break;
+          if (d < 4) {
+            m = mul(m, U.template load<double>(d, y.s));
+            y.s += off_rt(g, y, d, 1);
+            y.x[0] = y.s % g.L[0];
+            y.x[1] = (y.s / g.S[1]) % g.L[1];
+            y.x[2] = (y.s / g.S[2]) % g.L[2];
+            y.x[3] = y.s / g.S[3];
+          } else {
+            y.s += off_rt(g, y, d - 4, -1);
+            y.x[0] = y.s % g.L[0];
+            y.x[1] = (y.s / g.S[1]) % g.L[1];
+            y.x[2] = (y.s / g.S[2]) % g.L[2];
+            y.x[3] = y.s / g.S[3];
+            m = mul_nd(m, U.template load<double>(d - 4, y.s));
+          }
+        }
+        const int kd = tab->kind[q];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k == kd) add_to(sum[k], m);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+          part[k * 18 + 2 * e][lane] = sum[k].e[e].r;
+          part[k * 18 + 2 * e + 1][lane] = sum[k].e[e].i;
+        }
+      __syncthreads();
+      for (int comp = lane; comp < 72; comp += 64) {
+        double a = 0.;
+        for (int l = 0; l < 64; ++l) a += part[comp][l];
+        tot[comp] = a;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        M3<double> X[4];
+        for (int k = 0; k < 4; ++k)
+          for (int e = 0; e < 9; ++e) X[k].e[e] = {tot[k * 18 + 2 * e], tot[k * 18 + 2 * e + 1]};
+        // plaquette and rectangle sums are used daggered (get_staple_dagger,
+        // get_rectangular_dagger); chair and twist undaggered (field-staple.h:149,164)
+        X[0] = dag(X[0]);
+        X[1] = dag(X[1]);
+        M3<double> v = U.template load<double>(mu, s);
+        const uint32_t link = (uint32_t)s * 4u + (uint32_t)mu;
+        for (int n = 0; n < 2; ++n) {
+          const u32x4 w0 = rng_draw(seed, RNG_DEMON, (uint64_t)sweep, link, 3 * n);
+          const u32x4 w1 = rng_draw(seed, RNG_DEMON, (uint64_t)sweep, link, 3 * n + 1);
+          const u32x4 w2 = rng_draw(seed, RNG_DEMON, (uint64_t)sweep, link, 3 * n + 2);
+          // random_su3_from_su2(0.9) (field-matrix.h:36-65)
+          const double a0 = 0.9 + (1. - 0.9) * rng_u01(w0.x, w0.y);
+          double a1 = -1. + 2. * rng_u01(w0.z, w0.w);
+          double a2 = -1. + 2. * rng_u01(w1.x, w1.y);
+          double a3 = -1. + 2. * rng_u01(w1.z, w1.w);
+          const double nrm = sqrt((1. - a0 * a0) / (a1 * a1 + a2 * a2 + a3 * a3));
+          a1 *= nrm;
+          a2 *= nrm;
+          a3 *= nrm;
+          M3<double> m = unit3<double>();
+          switch (w2.x % 3) {
+            case 0: m(1, 1) = {a0, a3}; m(1, 2) = {a2, a1}; m(2, 1) = {-a2, a1}; m(2, 2) = {a0, -a3}; break;
+            case 1: m(0, 0) = {a0, a3}; m(0, 2) = {a2, a1}; m(2, 0) = {-a2, a1}; m(2, 2) = {a0, -a3}; break;
+            default: m(0, 0) = {a0, a3}; m(0, 1) = {a2, a1}; m(1, 0) = {-a2, a1}; m(1, 1) = {a0, -a3}; break;
+          }
+          const M3<double> mv = mul(m, v);
+          double dS[4];
+          bool ok = true;
+          for (int k = 0; k < 4; ++k) {
+            dS[k] = -retrace_mul(mv, X[k]) + retrace_mul(v, X[k]);
+            ok = ok && (E[k] - dS[k] > -emax) && (E[k] - dS[k] < emax);
+          }
+          if (ok) {
+            v = mv;
+            for (int k = 0; k < 4; ++k) E[k] -= dS[k];
+            ++acc;
+          } else {
+            ++rej;
+          }
+        }
+        U.store(mu, s, v);
+      }
+      // make lane 0's link store visible to every lane's next loads
+      __threadfence();
+      __syncthreads();
+    }
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 4; ++k) st->E[k] = E[k];
+    st->acc = acc;
+    st->rej = rej;
+  }
+}
+
+}  // namespace
+
+extern "C" int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
+                                        long acc_rej[2]) {
+  if (!U || !E || !acc_rej || nsweeps < 0) return MAG_ERR_ARG;
+  mag_ctx c = U->ctx;
+  if (U->kind != MAG_LINK18 || U->prec != MAG_F64)
+    return fail(c, MAG_ERR_UNSUPPORTED, "the demon runs on FP64 18-real links");
+  DemonTable* ht = new DemonTable();
+  build_table(*ht);
+  DemonTable* dt = nullptr;
+  DemonState* ds = nullptr;
+  int r = dev_alloc(c, (void**)&dt, sizeof(DemonTable));
+  if (!r) r = dev_alloc(c, (void**)&ds, sizeof(DemonState));
+  DemonState h{};
+  for (int k = 0; k < 4; ++k) h.E[k] = E[k];
+  if (!r && (hipMemcpyAsync(dt, ht, sizeof(DemonTable), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+             hipMemcpyAsync(ds, &h, sizeof h, hipMemcpyHostToDevice, c->stream) != hipSuccess))
+    r = fail(c, MAG_ERR_DEVICE, "demon upload failed");
+  for (int sw = 0; sw < nsweeps && !r; ++sw) {
+    KTimer t(c, "demon");
+    hipLaunchKernelGGL(k_demon_sweep, dim3(1), dim3(64), 0, c->stream, LinkView<double, 18>{(double*)U->p, U->g.V},
+                       U->g, dt, ds, E_max, seed, sw);
+    if (hipGetLastError() != hipSuccess) r = fail(c, MAG_ERR_DEVICE, "demon launch failed");
+  }
+  if (!r && (hipMemcpyAsync(&h, ds, sizeof h, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+             hipStreamSynchronize(c->stream) != hipSuccess))
+    r = fail(c, MAG_ERR_DEVICE, "demon download failed");
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(c, dt, dt ? sizeof(DemonTable) : 0);
+  dev_free(c, ds, ds ? sizeof(DemonState) : 0);
+  delete ht;
+  if (r) return r;
+  for (int k = 0; k < 4; ++k) E[k] = h.E[k];
+  acc_rej[0] = (long)h.acc;
+  acc_rej[1] = (long)h.rej;
+  return MAG_OK;
+}

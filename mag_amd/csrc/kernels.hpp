@@ -1,0 +1,774 @@
+// gfx950 kernels of the constrained-HMC path. One thread per link (x, mu):
+// blockIdx.x runs over 256-site blocks of one direction and blockIdx.y is the
+// direction mu, so mu and every lattice extent/stride derived from it are
+// wavefront-uniform (scalar registers); only the site coordinates are
+// per-lane. xcd_block() remaps blockIdx.x so that each of the 8 XCDs walks
+// one contiguous slab of the lattice, keeping a link and its stencil
+// neighbours in the same XCD's L2.
+//
+// Template parameters: TS storage real, NC reals per stored link (12 or 18),
+// TC compute real. FP64 kernels (TS = TC = double, NC = 18) follow the
+// reference arithmetic; the FP32 molecular-dynamics kernels use TS = TC =
+// float with NC = 12. Hamiltonian reductions always accumulate in FP64.
+#pragma once
+#include <type_traits>
+
+#include "cayley.hpp"
+#include "lattice.hpp"
+#include "philox.hpp"
+
+namespace mag {
+
+constexpr int BLOCK = 256;
+
+MAG_D int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  if (nb % 8 != 0) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+// ======================================================= staples and forces
+enum ActionKind : int { ACT_WILSON = 0, ACT_IWASAKI = 1 };
+
+// Undaggered staple sum S of link (x, mu): c0 * (6 plaquette staples)
+// + c1 * (18 rectangle staples), the paths of get_staple_dagger
+// (field-hmc.h:226-240) and get_rectangular_dagger (field-hmc.h:137-201),
+// with c0 = 1 - 8 c1 (field-hmc.h:262-269). For Wilson, c0 = 1 and no
+// rectangles. The reference's Sigma^+ is S^+.
+//
+// Rectangles share sub-products per plane nu: the three paths that end in
+// U_nu(x+mu)^+ (plaquette, 1x2 tall, 2x1 backward) are summed before the
+// last multiply, as are the three that end in U_nu(x-nu+mu); the 2x1 forward
+// rectangles of every plane share the trailing U_mu(x+mu)^+, applied once
+// after the nu loop. 62 instead of 85 3x3 products per link.
+template <int ACT, class TS, int NC, class TC>
+MAG_D M3<TC> staple_sum(const LinkView<TS, NC>& u, const Geom& g, const Site& p, int mu, TC c0, TC c1) {
+  M3<TC> S = zero3<TC>();
+  M3<TC> R = zero3<TC>();
+  const int xpm = p.s + off_rt(g, p, mu, 1);
+  const int xmm = p.s + off_rt(g, p, mu, -1);
+  const int xp2m = p.s + off_rt(g, p, mu, 2);
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const int nu = i + (i >= mu ? 1 : 0);
+    const int opn = off_rt(g, p, nu, 1), omn = off_rt(g, p, nu, -1);
+    const int xpn = p.s + opn, xmn = p.s + omn;
+    if (ACT == ACT_WILSON) {
+      {
+        M3<TC> A = mul(u.template load<TC>(nu, p.s), u.template load<TC>(mu, xpn));
+        mul_nd_acc(S, A, u.template load<TC>(nu, xpm));
+      }
+      {
+        M3<TC> B = mul_dn(u.template load<TC>(nu, xmn), u.template load<TC>(mu, xmn));
+        mul_acc(S, B, u.template load<TC>(nu, xpm + omn));
+      }
+    } else {
+      const int op2n = off_rt(g, p, nu, 2), om2n = off_rt(g, p, nu, -2);
+      // ---- upper side of the plane
+      {
+        const M3<TC> Um_xpn = u.template load<TC>(mu, xpn);
+        // T3 = U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+
+        M3<TC> T = mul(u.template load<TC>(nu, xpn), u.template load<TC>(mu, p.s + op2n));
+        T = mul_nd(T, u.template load<TC>(nu, xpm + opn));
+        // tmp = c0 U_mu(x+nu) + c1 T3
+        M3<TC> tmp = scale(Um_xpn, c0);
+        axpy(tmp, c1, T);
+        const M3<TC> Un_x = u.template load<TC>(nu, p.s);
+        M3<TC> G = mul(Un_x, tmp);
+        // R1' = U_nu(x) U_mu(x+nu) U_mu(x+nu+mu) U_nu(x+2mu)^+
+        {
+          M3<TC> A = mul(Un_x, Um_xpn);
+          A = mul(A, u.template load<TC>(mu, xpm + opn));
+          mul_nd_acc(R, A, u.template load<TC>(nu, xp2m));
+        }
+        // T5 = U_mu(x-mu)^+ U_nu(x-mu) U_mu(x-mu+nu); G += c1 T5 U_mu(x+nu)
+        {
+          M3<TC> T5 = mul_dn(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm));
+          T5 = scale(mul(T5, u.template load<TC>(mu, xmm + opn)), c1);
+          mul_acc(G, T5, Um_xpn);
+        }
+        mul_nd_acc(S, G, u.template load<TC>(nu, xpm));
+      }
+      // ---- lower side of the plane
+      {
+        const M3<TC> Um_xmn = u.template load<TC>(mu, xmn);
+        // T4 = U_nu(x-2nu)^+ U_mu(x-2nu) U_nu(x-2nu+mu)
+        M3<TC> T = mul_dn(u.template load<TC>(nu, p.s + om2n), u.template load<TC>(mu, p.s + om2n));
+        T = mul(T, u.template load<TC>(nu, xpm + om2n));
+        M3<TC> tmp = scale(Um_xmn, c0);
+        axpy(tmp, c1, T);
+        const M3<TC> Un_xmn = u.template load<TC>(nu, xmn);
+        M3<TC> G = mul_dn(Un_xmn, tmp);
+        // R2' = U_nu(x-nu)^+ U_mu(x-nu) U_mu(x-nu+mu) U_nu(x-nu+2mu)
+        {
+          M3<TC> B = mul_dn(Un_xmn, Um_xmn);
+          B = mul(B, u.template load<TC>(mu, xpm + omn));
+          mul_acc(R, B, u.template load<TC>(nu, xp2m + omn));
+        }
+        // T6 = U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu); G += c1 T6 U_mu(x-nu)
+        {
+          M3<TC> T6 = mul_dd(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm + omn));
+          T6 = scale(mul(T6, u.template load<TC>(mu, xmm + omn)), c1);
+          mul_acc(G, T6, Um_xmn);
+        }
+        mul_acc(S, G, u.template load<TC>(nu, xpm + omn));
+      }
+    }
+  }
+  if (ACT == ACT_IWASAKI) {
+    R = scale(R, c1);
+    mul_nd_acc(S, R, u.template load<TC>(mu, xpm));
+  }
+  return S;
+}
+
+// Output of a force evaluation (chosen at run time; wave-uniform).
+enum ForceOut : int {
+  FOUT_F = 0,     // store F (8 algebra components)
+  FOUT_PADD = 1,  // P += eps F           (evolve_momentum, field-hmc.h:427-440)
+  FOUT_UAUX = 2,  // Uaux = e^{i eps F} U (FG auxiliary step, field-hmc.h:546-547)
+};
+
+template <class TS, int NC>
+struct ForceArgs {
+  Geom g;
+  LinkView<TS, NC> U;     // links the force is evaluated on
+  AlgView<TS> F;          // FOUT_F
+  AlgView<TS> P;          // FOUT_PADD
+  LinkView<TS, NC> Uaux;  // FOUT_UAUX
+  double beta, c1;
+  int mag;
+  int out;
+  double eps;
+};
+
+template <class TS, int NC, class TC>
+MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC>& Ux, TC f[8]) {
+  if (a.out == FOUT_F) {
+    a.F.store(mu, s, f);
+  } else if (a.out == FOUT_PADD) {
+    TC p[8];
+    a.P.load(mu, s, p);
+    const TC e = TC(a.eps);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) p[k] = mfma(e, f[k], p[k]);
+    a.P.store(mu, s, p);
+  } else {
+    const TC e = TC(a.eps);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] *= e;
+    const M3<TC> Q = alg_to_mat(f);
+    a.Uaux.store(mu, s, mul(expiQ(Q), Ux));
+  }
+}
+
+// get_force (field-hmc.h:339-425): F = (i beta/3) TA(W) with W = U S^+ for
+// free links, W = U(x+mu) S(x+mu)^+ - S(x)^+ U(x) for constraint classes 1 and
+// 10, and F = 0 for class 100. Stored as F_a = -(beta/3) Im Tr(T_a W).
+template <int ACT, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_gauge_force(ForceArgs<TS, NC> a) {
+  const int s = xcd_block() * BLOCK + threadIdx.x;
+  if (s >= a.g.V) return;
+  const int mu = blockIdx.y;
+  const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
+  const TC c0 = TC(1) - TC(8) * c1;
+  const Site p = site_of(a.g, s);
+  const int cls = link_class_rt(p, mu, a.mag);
+  const M3<TC> Ux = a.U.template load<TC>(mu, s);
+  M3<TC> W = zero3<TC>();
+  if (cls != 100) {
+    // class 0: one staple at x; classes 1, 10: staples at x and x+mu
+    const int nsite = (cls == 0) ? 1 : 2;
+    const int ys = s + off_rt(a.g, p, mu, 1);
+#pragma unroll 1
+    for (int k = 0; k < nsite; ++k) {
+      const Site q = (k == 0) ? p : site_of(a.g, ys);
+      const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, q, mu, c0, c1);
+      if (cls == 0) W = mul_nd(Ux, S);
+      else if (k == 0) W = sub(W, mul_dn(S, Ux));
+      else mul_nd_acc(W, a.U.template load<TC>(mu, ys), S);
+    }
+  }
+  TC f[8];
+  force_alg_from(W, f);
+  const TC fac = TC(-a.beta / 3.);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) f[k] *= fac;
+  force_epilogue<TS, NC, TC>(a, mu, s, Ux, f);
+}
+
+// ============================================================ link updates
+// evolve_gauge_field (field-hmc.h:442-482) with the constraint classes:
+// 0: U <- e^{i eps P(x)} U; 10: U <- U e^{-i eps P(x)};
+// 1: U <- e^{i eps P(x-mu)} U e^{-i eps P(x)}; 100: U <- e^{i eps P(x-mu)} U.
+// Uout may alias Uin (each thread reads and writes only its own link).
+template <class TS, int NC>
+struct UpdateArgs {
+  Geom g;
+  LinkView<TS, NC> Uin, Uout;
+  AlgView<TS> P;
+  double eps;
+  int mag;
+};
+
+template <class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= a.g.V) return;
+  const int mu = blockIdx.y;
+  const Site p = site_of(a.g, s);
+  const int cls = link_class_rt(p, mu, a.mag);
+  const TC e = TC(a.eps);
+  M3<TC> U = a.Uin.template load<TC>(mu, s);
+  TC h[8];
+  if (cls == 0 || cls == 1 || cls == 10) {
+    a.P.load(mu, s, h);
+    const TC sg = (cls == 0) ? e : -e;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) h[k] *= sg;
+    const M3<TC> E = expiQ(alg_to_mat(h));
+    U = (cls == 0) ? mul(E, U) : mul(U, E);
+  }
+  if (cls == 1 || cls == 100) {
+    a.P.load(mu, s + off_rt(a.g, p, mu, -1), h);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) h[k] *= e;
+    U = mul(expiQ(alg_to_mat(h)), U);
+  }
+  a.Uout.store(mu, s, U);
+}
+
+// P += eps m F (field-hmc.h:427-440), for the unfused path.
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_mom_update(AlgView<TS> P, AlgView<TS> F, int V, double eps) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= V) return;
+  const int mu = blockIdx.y;
+  double p[8], f[8];
+  P.load(mu, s, p);
+  F.load(mu, s, f);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) p[k] = mfma(eps, f[k], p[k]);
+  P.store(mu, s, p);
+}
+
+// ============================================================ stout chain
+// Forward level k (hep-lat/0311018 eqs. 1-3; get_Q, field-multi.h:32-39,
+// with the 6-plaquette staple): C = rho S, Q = TA(S U^+) (-i rho),
+// U_{k+1} = e^{iQ} U_k. Stores Q (8 components), C (3x3) and U_{k+1}.
+template <class TSI, int NCI, class TSO, int NCO>
+struct StoutFwdArgs {
+  Geom g;
+  LinkView<TSI, NCI> U;
+  AlgView<TSO> Q;
+  MatView<TSO> C;
+  LinkView<TSO, NCO> Unext;
+  double rho;
+  int store_qc;  // 0 when only the smeared field is wanted (Hamiltonian)
+};
+
+template <class TSI, int NCI, class TSO, int NCO, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) {
+  const int s = xcd_block() * BLOCK + threadIdx.x;
+  if (s >= a.g.V) return;
+  const int mu = blockIdx.y;
+  const Site p = site_of(a.g, s);
+  const M3<TC> S = staple_sum<ACT_WILSON, TSI, NCI, TC>(a.U, a.g, p, mu, TC(1), TC(0));
+  const M3<TC> Ux = a.U.template load<TC>(mu, s);
+  TC q[8];
+  force_alg_from(mul_nd(S, Ux), q);  // -i TA(S U^+)
+  const TC rho = TC(a.rho);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) q[k] *= rho;
+  const M3<TC> Q = alg_to_mat(q);
+  a.Unext.store(mu, s, mul(expiQ(Q), Ux));
+  if (a.store_qc) {
+    a.Q.store(mu, s, q);
+    a.C.store(mu, s, scale(S, rho));
+  }
+}
+
+// Lambda step of level k for one link, given Z' = dS/dU_{k+1} there
+// (compute_Lambda, field-multi.h:180-269, eqs. 72-73):
+//   Lambda = HT(Gamma), Y_k = -i rho U_k^+ Lambda,
+//   W_k = Z' e^{iQ_k} + i C_k^+ Lambda  (the local part of Z_k, eq. 75).
+template <class TS, int NC, class TC>
+MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS, NC>& U, const AlgView<TS>& Qv,
+                             const MatView<TS>& Cv, const MatView<TS>& Yout, const MatView<TS>& Wout, TC rho) {
+  const M3<TC> Uk = U.template load<TC>(mu, s);
+  const M3<TC> Q = Qv.template load_mat<TC>(mu, s);
+  const M3<TC> X = mul(Uk, Zp);
+  TC lam[8];
+  M3<TC> eQ;
+  ch_lambda(Q, X, lam, eQ);
+  const M3<TC> Lm = alg_to_mat(lam);
+  M3<TC> W = mul(Zp, eQ);
+  // + i C^+ Lambda
+  const M3<TC> CL = mul_dn(Cv.template load<TC>(mu, s), Lm);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) W.e[k] = W.e[k] + cpx<TC>{-CL.e[k].i, CL.e[k].r};
+  Wout.store(mu, s, W);
+  // Y = -i rho U^+ Lambda
+  const M3<TC> UL = mul_dn(Uk, Lm);
+  M3<TC> Y;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Y.e[k] = cpx<TC>{rho * UL.e[k].i, -rho * UL.e[k].r};
+  Yout.store(mu, s, Y);
+}
+
+// Top of the chain: Z_n = -(beta/3) S(U_n)^+ (derivative of the gauge action
+// on the smeared field), then the Lambda step of level n-1.
+template <class TS, int NC>
+struct StoutTopArgs {
+  Geom g;
+  LinkView<TS, NC> Un;      // smeared links U_n
+  LinkView<TS, NC> Ulo;     // U_{n-1}
+  AlgView<TS> Qlo;
+  MatView<TS> Clo;
+  MatView<TS> Yout, Wout;
+  double beta, c1, rho;
+};
+
+template <int ACT, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_top(StoutTopArgs<TS, NC> a) {
+  const int s = xcd_block() * BLOCK + threadIdx.x;
+  if (s >= a.g.V) return;
+  const int mu = blockIdx.y;
+  const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
+  const TC c0 = TC(1) - TC(8) * c1;
+  const Site p = site_of(a.g, s);
+  const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.Un, a.g, p, mu, c0, c1);
+  const M3<TC> Z = scale(dag(S), TC(-a.beta / 3.));
+  stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Yout, a.Wout, TC(a.rho));
+}
+
+// Backward level k (eq. 75): Z_k = W_k + sum over nu of the six
+// Lambda-inserted staple terms, with Y = -i rho U^+ Lambda:
+//   U_nu(x+mu) [U_mu(x+nu)^+ Y_nu(x) + Y_mu(x+nu) U_nu(x)^+]
+// + U_nu(x-nu+mu)^+ [U_mu(x-nu)^+ Y_nu(x-nu)^+ + Y_mu(x-nu) U_nu(x-nu)]
+// + Y_nu(x+mu)^+ (U_nu(x) U_mu(x+nu))^+ + Y_nu(x+mu-nu) U_mu(x-nu)^+ U_nu(x-nu)
+// Then either the Lambda step of level k-1 (LAST = false) or, at k = 0, the
+// thin-link force F_a = Im Tr(T_a U Z_0) with a force epilogue.
+template <class TS, int NC>
+struct StoutBwdArgs {
+  Geom g;
+  LinkView<TS, NC> U;   // U_k
+  MatView<TS> Y, W;     // level k
+  // k > 0: Lambda step of level k-1
+  LinkView<TS, NC> Ulo;
+  AlgView<TS> Qlo;
+  MatView<TS> Clo;
+  MatView<TS> Yout, Wout;
+  double rho;
+  // k == 0: force output
+  ForceArgs<TS, NC> f;
+  MatView<TS> Zout;  // constrained runs: Z_0 is stored for k_force_from_Z
+};
+
+template <class TS, int NC, class TC>
+MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int mu) {
+  const Geom& g = a.g;
+  M3<TC> Z = a.W.template load<TC>(mu, p.s);
+  const int xpm = p.s + off_rt(g, p, mu, 1);
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const int nu = i + (i >= mu ? 1 : 0);
+    const int opn = off_rt(g, p, nu, 1), omn = off_rt(g, p, nu, -1);
+    const int xpn = p.s + opn, xmn = p.s + omn;
+    {
+      const M3<TC> Um_xpn = a.U.template load<TC>(mu, xpn);
+      const M3<TC> Un_x = a.U.template load<TC>(nu, p.s);
+      M3<TC> T = mul_dn(Um_xpn, a.Y.template load<TC>(nu, p.s));
+      mul_nd_acc(T, a.Y.template load<TC>(mu, xpn), Un_x);
+      mul_acc(Z, a.U.template load<TC>(nu, xpm), T);
+      // Y_nu(x+mu)^+ (U_nu(x) U_mu(x+nu))^+
+      const M3<TC> A = mul(Un_x, Um_xpn);
+      Z = add(Z, mul_dd(a.Y.template load<TC>(nu, xpm), A));
+    }
+    {
+      const M3<TC> Um_xmn = a.U.template load<TC>(mu, xmn);
+      const M3<TC> Un_xmn = a.U.template load<TC>(nu, xmn);
+      M3<TC> T = mul_nd(dag(Um_xmn), a.Y.template load<TC>(nu, xmn));
+      mul_acc(T, a.Y.template load<TC>(mu, xmn), Un_xmn);
+      mul_dn_acc(Z, a.U.template load<TC>(nu, xpm + omn), T);
+      // Y_nu(x+mu-nu) U_mu(x-nu)^+ U_nu(x-nu)
+      const M3<TC> B = mul_dn(Um_xmn, Un_xmn);
+      mul_acc(Z, a.Y.template load<TC>(nu, xpm + omn), B);
+    }
+  }
+  return Z;
+}
+
+template <bool LAST, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_bwd(StoutBwdArgs<TS, NC> a) {
+  const int s = xcd_block() * BLOCK + threadIdx.x;
+  if (s >= a.g.V) return;
+  const int mu = blockIdx.y;
+  const Site p = site_of(a.g, s);
+  const M3<TC> Z = stout_six_terms<TS, NC, TC>(a, p, mu);
+  if (!LAST) {
+    stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Yout, a.Wout, TC(a.rho));
+  } else if (a.f.mag) {
+    a.Zout.store(mu, s, Z);
+  } else {
+    const M3<TC> Ux = a.U.template load<TC>(mu, s);
+    TC f[8];
+    force_alg_from(mul(Ux, Z), f);
+    force_epilogue<TS, NC, TC>(a.f, mu, s, Ux, f);
+  }
+}
+
+// Constrained runs with smearing: F from a stored Z_0 with the classes of
+// field-hmc.h:353-378: W = U Z (0), U(x+mu) Z(x+mu) - Z U (1, 10), 0 (100);
+// F_a = Im Tr(T_a W).
+template <class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_force_from_Z(ForceArgs<TS, NC> a, MatView<TS> Z) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= a.g.V) return;
+  const int mu = blockIdx.y;
+  const Site p = site_of(a.g, s);
+  const int cls = link_class_rt(p, mu, a.mag);
+  M3<TC> W = zero3<TC>();
+  const M3<TC> Ux = a.U.template load<TC>(mu, s);
+  if (cls == 0) {
+    W = mul(Ux, Z.template load<TC>(mu, s));
+  } else if (cls != 100) {
+    const int ys = s + off_rt(a.g, p, mu, 1);
+    W = sub(mul(a.U.template load<TC>(mu, ys), Z.template load<TC>(mu, ys)), mul(Z.template load<TC>(mu, s), Ux));
+  }
+  TC f[8];
+  force_alg_from(W, f);
+  a.F.store(mu, s, f);
+}
+
+// ============================================================ RNG kernels
+// init_momentum (field-hmc.h:646-673): P = sum_a T_a g_a, g_a ~ N(0, sqrt m),
+// drawn from the Philox momentum stream of link (s*4+mu), epoch = trajectory.
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_mom_refresh(AlgView<TS> P, int V, uint64_t seed, uint64_t epoch,
+                                                       double sig) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= V) return;
+  const int mu = blockIdx.y;
+  const uint32_t link = (uint32_t)s * 4u + (uint32_t)mu;
+  double g[8];
+#pragma unroll
+  for (uint32_t d = 0; d < 4; ++d) rng_gauss2(rng_draw(seed, RNG_MOMENTUM, epoch, link, d), g[2 * d], g[2 * d + 1]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) g[k] *= sig;
+  P.store(mu, s, g);
+}
+
+// Haar-random SU(3) links (Mezzadri 2007): Gram-Schmidt of a complex Gaussian
+// matrix, det phase divided out.
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_haar(LinkView<TS, NC> U, int V, uint64_t seed) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= V) return;
+  const int mu = blockIdx.y;
+  const uint32_t link = (uint32_t)s * 4u + (uint32_t)mu;
+  M3<double> z;
+#pragma unroll
+  for (uint32_t d = 0; d < 9; ++d) {
+    double g0, g1;
+    rng_gauss2(rng_draw(seed, RNG_HAAR, 0, link, d), g0, g1);
+    z.e[d] = {g0, g1};
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int q = 0; q < r; ++q) {
+      cpx<double> c = {0., 0.};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) c = c + cconj(z(q, j)) * z(r, j);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) z(r, j) = z(r, j) - c * z(q, j);
+    }
+    double n = 0.;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) n += z(r, j).r * z(r, j).r + z(r, j).i * z(r, j).i;
+    n = 1. / sqrt(n);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) z(r, j) = n * z(r, j);
+  }
+  const cpx<double> d = z(0, 0) * (z(1, 1) * z(2, 2) - z(1, 2) * z(2, 1)) -
+                        z(0, 1) * (z(1, 0) * z(2, 2) - z(1, 2) * z(2, 0)) +
+                        z(0, 2) * (z(1, 0) * z(2, 1) - z(1, 1) * z(2, 0));
+  const double phi = atan2(d.i, d.r);
+  z = cscale(z, cpx<double>{cos(-phi / 3.), sin(-phi / 3.)});
+  U.store(mu, s, z);
+}
+
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_unit(LinkView<TS, NC> U, int V) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= V) return;
+  U.store(blockIdx.y, s, unit3<double>());
+}
+
+// ============================================================ reductions
+// Block sum in a fixed order: wavefront shuffles, then wave partials in LDS.
+// The result depends only on the values and the block shape, never on timing.
+MAG_D double block_sum(double v) {
+  __shared__ double red[BLOCK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+  }
+  __syncthreads();
+  return t;
+}
+MAG_D double block_max(double v) {
+  __shared__ double red[BLOCK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) t = fmax(t, red[w]);
+  }
+  __syncthreads();
+  return t;
+}
+
+// Per-site plaquette and rectangle Re Tr sums in FP64 (get_plaq,
+// field-matrix.h:202-221; get_rectangular, field-matrix.h:249-297), one
+// partial pair per block: part[2b] = plaquettes, part[2b+1] = rectangles.
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_action_partial(LinkView<TS, NC> U, Geom g, double* part, int rect) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  double sp = 0., sr = 0.;
+  if (s < g.V) {
+    const Site p = site_of(g, s);
+#pragma unroll 1
+    for (int pl = 0; pl < 6; ++pl) {
+      // planes (mu, nu) with nu < mu: (1,0) (2,0) (2,1) (3,0) (3,1) (3,2)
+      const int mu = pl < 1 ? 1 : (pl < 3 ? 2 : 3);
+      const int nu = pl - (mu * (mu - 1)) / 2;
+      const int xpm = s + off_rt(g, p, mu, 1);
+      const int xpn = s + off_rt(g, p, nu, 1);
+      // U_mu(x) U_nu(x+mu) [U_nu(x) U_mu(x+nu)]^+
+      const M3<double> Um = U.template load<double>(mu, s);
+      const M3<double> Un = U.template load<double>(nu, s);
+      const M3<double> m1 = mul(Um, U.template load<double>(nu, xpm));
+      const M3<double> n1 = mul(Un, U.template load<double>(mu, xpn));
+      sp += retrace_mul_nd(m1, n1);
+      if (rect) {
+        const int xpmpn = xpm + off_rt(g, p, nu, 1);
+        // 1x2 (tall in nu): U_nu(x) U_nu(x+nu) U_mu(x+2nu) [U_mu(x) U_nu(x+mu) U_nu(x+mu+nu)]^+
+        {
+          const M3<double> a = mul(mul(Un, U.template load<double>(nu, xpn)),
+                                   U.template load<double>(mu, s + off_rt(g, p, nu, 2)));
+          const M3<double> b = mul(m1, U.template load<double>(nu, xpmpn));
+          sr += retrace_mul_nd(a, b);
+        }
+        // 2x1 (long in mu): U_nu(x) U_mu(x+nu) U_mu(x+mu+nu) [U_mu(x) U_mu(x+mu) U_nu(x+2mu)]^+
+        {
+          const M3<double> a = mul(n1, U.template load<double>(mu, xpmpn));
+          const M3<double> b = mul(mul(Um, U.template load<double>(mu, xpm)),
+                                   U.template load<double>(nu, s + off_rt(g, p, mu, 2)));
+          sr += retrace_mul_nd(a, b);
+        }
+      }
+    }
+  }
+  const double tp = block_sum(sp);
+  const double tr = block_sum(sr);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = tp;
+    part[2 * blockIdx.x + 1] = tr;
+  }
+}
+
+// Sum p_a^2 over links of class != 100 (field-hmc.h:596-629), FP64.
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_kinetic_partial(AlgView<TS> P, Geom g, int mag, double* part) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  double acc = 0.;
+  if (s < g.V) {
+    const Site p = site_of(g, s);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      if (link_class_rt(p, mu, mag) == 100) continue;
+      double h[8];
+      P.load(mu, s, h);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc = mfma(h[k], h[k], acc);
+    }
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// reunitarize (field-matrix.h:83-100): project every link whose first two
+// rows are not all zero, FP64 arithmetic; max Frobenius change per block.
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_reunitarize(LinkView<TS, NC> U, int V, double* part) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  const int mu = blockIdx.y;
+  double dev = 0.;
+  if (s < V) {
+    M3<double> m = U.template load<double>(mu, s);
+    const M3<double> old = m;
+    bool all_zero = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) all_zero = all_zero && m.e[k].r == 0. && m.e[k].i == 0.;
+    if (!all_zero) unitarize(m);
+    double d = 0.;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const double dr = m.e[k].r - old.e[k].r, di = m.e[k].i - old.e[k].i;
+      d += dr * dr + di * di;
+    }
+    // the row-2 entries of a 12-real link are rebuilt on every load, so
+    // they carry no stored deviation
+    dev = sqrt(d);
+    U.store(mu, s, m);
+  }
+  const double t = block_max(dev);
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// Wilson loops (field-wloop.h:26-98): Re Tr of the R x T loop for every
+// permutation of r, per site, FP64.
+struct WloopArgs {
+  Geom g;
+  int nperm;
+  int r[6][3];
+  int T;
+};
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_wilson_loop(LinkView<TS, NC> U, WloopArgs a, double* part) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  double acc = 0.;
+  if (s < a.g.V) {
+    for (int q = 0; q < a.nperm; ++q) {
+      Site y = site_of(a.g, s);
+      M3<double> m = unit3<double>();
+      auto fwd = [&](int d) {
+        m = mul(m, U.template load<double>(d, y.s));
+        y.s += off_rt(a.g, y, d, 1);
+        y.x[d] = (y.x[d] + 1 == a.g.L[d]) ? 0 : y.x[d] + 1;
+      };
+      auto bwd = [&](int d) {
+        y.s += off_rt(a.g, y, d, -1);
+        y.x[d] = (y.x[d] == 0) ? a.g.L[d] - 1 : y.x[d] - 1;
+        m = mul_nd(m, U.template load<double>(d, y.s));
+      };
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < a.r[q][i]; ++j) fwd(i);
+      for (int j = 0; j < a.T; ++j) fwd(3);
+      for (int i = 2; i >= 0; --i)
+        for (int j = 0; j < a.r[q][i]; ++j) bwd(i);
+      for (int j = 0; j < a.T; ++j) bwd(3);
+      acc += retrace(m);
+    }
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Final reduction of block partials, one block, fixed order: each thread
+// folds a strided slice, then the block tree. OP 0 = sum, 1 = max.
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_final_reduce(const double* part, int n, int stride, double* out) {
+  double acc = 0.;
+  for (int i = threadIdx.x; i < n; i += BLOCK) {
+    const double v = part[(size_t)i * stride];
+    acc = OP ? fmax(acc, v) : acc + v;
+  }
+  const double t = OP ? block_max(acc) : block_sum(acc);
+  if (threadIdx.x == 0) *out = t;
+}
+
+// ============================================================ conversions
+template <class TSI, int NCI, class TSO, int NCO>
+__global__ __launch_bounds__(BLOCK) void k_convert_links(LinkView<TSI, NCI> in, LinkView<TSO, NCO> out, int V) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= V) return;
+  const int mu = blockIdx.y;
+  out.store(mu, s, in.template load<double>(mu, s));
+}
+
+// max over links of |A - B| (Frobenius), per-block partials
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_max_diff(LinkView<TS, NC> A, LinkView<TS, NC> B, int V, double* part) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  const int mu = blockIdx.y;
+  double dev = 0.;
+  if (s < V) {
+    const M3<double> a = A.template load<double>(mu, s), b = B.template load<double>(mu, s);
+    double d = 0.;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const double dr = a.e[k].r - b.e[k].r, di = a.e[k].i - b.e[k].i;
+      d += dr * dr + di * di;
+    }
+    dev = sqrt(d);
+  }
+  const double t = block_max(dev);
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// P <- s P (momentum flip for reversibility checks)
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_alg_scale(AlgView<TS> P, int V, double sc) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= V) return;
+  const int mu = blockIdx.y;
+  double h[8];
+  P.load(mu, s, h);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) h[k] *= sc;
+  P.store(mu, s, h);
+}
+
+// Batched per-matrix test hooks (FP64 or FP32 arithmetic).
+template <class TC>
+__global__ __launch_bounds__(BLOCK) void k_expiQ_batch(const double* Q, double* out, long n) {
+  const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  M3<TC> q;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) q.e[k] = {TC(Q[18 * i + 2 * k]), TC(Q[18 * i + 2 * k + 1])};
+  const M3<TC> e = expiQ(q);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    out[18 * i + 2 * k] = double(e.e[k].r);
+    out[18 * i + 2 * k + 1] = double(e.e[k].i);
+  }
+}
+
+template <class TC>
+__global__ __launch_bounds__(BLOCK) void k_lambda_batch(const double* Q, const double* Sg, const double* U,
+                                                        double* out, long n) {
+  const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  M3<TC> q, sg, u;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    q.e[k] = {TC(Q[18 * i + 2 * k]), TC(Q[18 * i + 2 * k + 1])};
+    sg.e[k] = {TC(Sg[18 * i + 2 * k]), TC(Sg[18 * i + 2 * k + 1])};
+    u.e[k] = {TC(U[18 * i + 2 * k]), TC(U[18 * i + 2 * k + 1])};
+  }
+  TC lam[8];
+  M3<TC> eQ;
+  ch_lambda(q, mul(u, sg), lam, eQ);
+  const M3<TC> L = alg_to_mat(lam);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    out[18 * i + 2 * k] = double(L.e[k].r);
+    out[18 * i + 2 * k + 1] = double(L.e[k].i);
+  }
+}
+
+}  // namespace mag

@@ -1,0 +1,134 @@
+// Lattice geometry and structure-of-arrays field views in HBM.
+//
+// Sites are ordered lexicographically, s = x0 + L0 (x1 + L1 (x2 + L2 x3)),
+// the order of qlat's coordinate_from_index (used throughout the reference,
+// e.g. field-hmc.h:349). Each field is component-major per direction:
+//   real c of direction mu at site s  ->  p[(mu * NC + c) * V + s]
+// so a wavefront of 64 consecutive sites reads 64 consecutive reals of one
+// component (256 B in FP32): every load is fully coalesced.
+//   link fields   NC = 18 (full 3x3) or 12 (rows 0,1; row 2 = conj(r0 x r1))
+//   algebra fields (momenta, forces, stout Q, Lambda)  NC = 8, H = sum h_a T_a
+//   general matrix fields (stout C, Y, W, Z)          NC = 18
+#pragma once
+#include "su3.hpp"
+
+#define MAG_D __device__ __forceinline__
+
+namespace mag {
+
+struct Geom {
+  int L[4];
+  int S[4];  // strides 1, L0, L0 L1, L0 L1 L2
+  int V;
+};
+
+inline Geom make_geom(const int* L) {
+  Geom g;
+  int s = 1;
+  for (int i = 0; i < 4; ++i) {
+    g.L[i] = L[i];
+    g.S[i] = s;
+    s *= L[i];
+  }
+  g.V = s;
+  return g;
+}
+
+struct Site {
+  int s;
+  int x[4];
+};
+
+MAG_HD Site site_of(const Geom& g, int s) {
+  Site p;
+  p.s = s;
+  int r = s;
+  p.x[0] = r % g.L[0]; r /= g.L[0];
+  p.x[1] = r % g.L[1]; r /= g.L[1];
+  p.x[2] = r % g.L[2];
+  p.x[3] = r / g.L[2];
+  return p;
+}
+
+// Index offset of a shift by d (|d| <= 2) along direction mu, with periodic wrap.
+// Select a[i] from a 4-array without dynamic register indexing.
+MAG_HD int sel4(const int* a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3])); }
+
+MAG_HD int off_rt(const Geom& g, const Site& p, int mu, int d) {
+  const int L = sel4(g.L, mu), x = sel4(p.x, mu);
+  int y = x + d;
+  y = (y >= L) ? y - L : ((y < 0) ? y + L : y);
+  return (y - x) * sel4(g.S, mu);
+}
+
+// Constraint class of link (x, mu) for block size mag (field-hmc.h:309-337):
+// 0 free, 10 first, 1 middle, 100 last link of a coarse link.
+MAG_HD int link_class_rt(const Site& p, int mu, int mag) {
+  if (!mag) return 0;
+  bool c = true;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i != mu) c = c && (p.x[i] % mag == 0);
+  if (!c) return 0;
+  const int r = sel4(p.x, mu) % mag;
+  if (r == mag - 1) return 100;
+  if (r == 0) return 10;
+  return 1;
+}
+
+// ------------------------------------------------------------------ views
+template <class TS, int NC>
+struct LinkView {
+  TS* p;
+  int V;
+  template <class TC>
+  MAG_D M3<TC> load(int mu, int s) const {
+    const TS* b = p + (size_t)mu * NC * V + s;
+    M3<TC> m;
+#pragma unroll
+    for (int k = 0; k < NC / 2; ++k) {
+      m.e[k].r = TC(b[(size_t)(2 * k) * V]);
+      m.e[k].i = TC(b[(size_t)(2 * k + 1) * V]);
+    }
+    if (NC == 12) reconstruct_row2(m);
+    return m;
+  }
+  template <class TC>
+  MAG_D void store(int mu, int s, const M3<TC>& m) const {
+    TS* b = p + (size_t)mu * NC * V + s;
+#pragma unroll
+    for (int k = 0; k < NC / 2; ++k) {
+      b[(size_t)(2 * k) * V] = TS(m.e[k].r);
+      b[(size_t)(2 * k + 1) * V] = TS(m.e[k].i);
+    }
+  }
+};
+
+template <class TS>
+struct AlgView {
+  TS* p;
+  int V;
+  template <class TC>
+  MAG_D void load(int mu, int s, TC h[8]) const {
+    const TS* b = p + (size_t)mu * 8 * V + s;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) h[a] = TC(b[(size_t)a * V]);
+  }
+  template <class TC>
+  MAG_D void store(int mu, int s, const TC h[8]) const {
+    TS* b = p + (size_t)mu * 8 * V + s;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) b[(size_t)a * V] = TS(h[a]);
+  }
+  template <class TC>
+  MAG_D M3<TC> load_mat(int mu, int s) const {
+    TC h[8];
+    load(mu, s, h);
+    return alg_to_mat(h);
+  }
+};
+
+template <class TS>
+using MatView = LinkView<TS, 18>;
+
+}  // namespace mag

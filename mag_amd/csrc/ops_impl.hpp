@@ -1,0 +1,326 @@
+// Definitions of Ops<TS, NC>: the launch schedules of the force (thin or
+// stout-smeared), link updates, integrators and reductions. Included only by
+// the per-format instantiation units (inst_f64.hip, inst_f32.hip).
+#pragma once
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace mag {
+
+template <class TS, int NC>
+LinkView<TS, NC> lview(void* p, const Geom& g) { return LinkView<TS, NC>{(TS*)p, g.V}; }
+template <class TS>
+AlgView<TS> aview(void* p, const Geom& g) { return AlgView<TS>{(TS*)p, g.V}; }
+template <class TS>
+MatView<TS> mview(void* p, const Geom& g) { return MatView<TS>{(TS*)p, g.V}; }
+
+template <class TS, int NC>
+int Ops<TS, NC>::link_update(mag_ctx c, const Geom& g, void* Uin, void* Uout, void* P, double eps, int mag) {
+  UpdateArgs<TS, NC> a{g, lview<TS, NC>(Uin, g), lview<TS, NC>(Uout, g), aview<TS>(P, g), eps, mag};
+  {
+    KTimer t(c, "link_update");
+    hipLaunchKernelGGL((k_link_update<TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, a);
+  }
+  CHECK_LAUNCH(c);
+  return MAG_OK;
+}
+
+// Force of the (possibly smeared) action on U with one of the epilogues
+// OUT_F -> F, OUT_PADD -> P += eps F, OUT_UAUX -> Uaux = e^{i eps F} U.
+// Thin action: one kernel (k_gauge_force). Smeared action (n levels):
+// n forward kernels, the top kernel (action derivative + Lambda of level
+// n-1), n-1 backward kernels each fused with the next Lambda step, and the
+// last backward kernel fused with the force epilogue.
+template <class TS, int NC>
+int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act, int out, void* F, void* P,
+                       void* Uaux, double eps, Workspace* ws) {
+  int r;
+  // constrained runs cannot fuse the auxiliary link update (it needs F(x-mu))
+  if (act.mag && out == OUT_UAUX) {
+    if ((r = ws->need_md())) return r;
+    if ((r = force(c, g, U, act, OUT_F, ws->F, nullptr, nullptr, 0., ws))) return r;
+    return link_update(c, g, U, Uaux, ws->F, eps, act.mag);
+  }
+  ForceArgs<TS, NC> fa{};
+  fa.g = g;
+  fa.U = lview<TS, NC>(U, g);
+  fa.F = aview<TS>(F, g);
+  fa.P = aview<TS>(P, g);
+  fa.Uaux = lview<TS, NC>(Uaux, g);
+  fa.beta = act.beta;
+  fa.c1 = act.c1;
+  fa.mag = act.mag;
+  fa.out = out;
+  fa.eps = eps;
+  const bool iw = act.type == MAG_IWASAKI;
+  if (act.n_stout <= 0) {
+    if (iw) {
+      KTimer t(c, "force_iwasaki");
+      hipLaunchKernelGGL((k_gauge_force<ACT_IWASAKI, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fa);
+    } else {
+      KTimer t(c, "force_wilson");
+      hipLaunchKernelGGL((k_gauge_force<ACT_WILSON, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fa);
+    }
+    CHECK_LAUNCH(c);
+    return MAG_OK;
+  }
+  const int n = act.n_stout;
+  if ((r = ws->need_stout(n))) return r;
+  auto Ulev = [&](int k) { return k == 0 ? U : ws->Us[k - 1]; };
+  for (int k = 0; k < n; ++k) {
+    StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g), mview<TS>(ws->Cs[k], g),
+                                    lview<TS, NC>(ws->Us[k], g), act.rho, 1};
+    {
+      KTimer t(c, "stout_fwd");
+      hipLaunchKernelGGL((k_stout_fwd<TS, NC, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
+    }
+    CHECK_LAUNCH(c);
+  }
+  int cur = 0;
+  {
+    StoutTopArgs<TS, NC> tp{g,
+                            lview<TS, NC>(Ulev(n), g),
+                            lview<TS, NC>(Ulev(n - 1), g),
+                            aview<TS>(ws->Qs[n - 1], g),
+                            mview<TS>(ws->Cs[n - 1], g),
+                            mview<TS>(ws->Y[cur], g),
+                            mview<TS>(ws->W, g),
+                            act.beta,
+                            act.c1,
+                            act.rho};
+    KTimer t(c, "stout_top");
+    if (iw)
+      hipLaunchKernelGGL((k_stout_top<ACT_IWASAKI, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, tp);
+    else
+      hipLaunchKernelGGL((k_stout_top<ACT_WILSON, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, tp);
+  }
+  CHECK_LAUNCH(c);
+  for (int k = n - 1; k >= 1; --k) {
+    StoutBwdArgs<TS, NC> bw{};
+    bw.g = g;
+    bw.U = lview<TS, NC>(Ulev(k), g);
+    bw.Y = mview<TS>(ws->Y[cur], g);
+    bw.W = mview<TS>(ws->W, g);
+    bw.Ulo = lview<TS, NC>(Ulev(k - 1), g);
+    bw.Qlo = aview<TS>(ws->Qs[k - 1], g);
+    bw.Clo = mview<TS>(ws->Cs[k - 1], g);
+    bw.Yout = mview<TS>(ws->Y[1 - cur], g);
+    bw.Wout = mview<TS>(ws->W, g);
+    bw.rho = act.rho;
+    {
+      KTimer t(c, "stout_bwd");
+      hipLaunchKernelGGL((k_stout_bwd<false, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, bw);
+    }
+    CHECK_LAUNCH(c);
+    cur = 1 - cur;
+  }
+  StoutBwdArgs<TS, NC> bw{};
+  bw.g = g;
+  bw.U = lview<TS, NC>(U, g);
+  bw.Y = mview<TS>(ws->Y[cur], g);
+  bw.W = mview<TS>(ws->W, g);
+  bw.rho = act.rho;
+  bw.f = fa;
+  if (act.mag) {
+    if ((r = ws->need_Z())) return r;
+    bw.Zout = mview<TS>(ws->Z, g);
+  }
+  {
+    KTimer t(c, "stout_bwd_force");
+    hipLaunchKernelGGL((k_stout_bwd<true, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, bw);
+  }
+  CHECK_LAUNCH(c);
+  if (!act.mag) return MAG_OK;
+  // constrained: F from the stored Z_0 with the link classes
+  void* Fout = (out == OUT_F) ? F : nullptr;
+  if (!Fout) {
+    if ((r = ws->need_md())) return r;
+    Fout = ws->F;
+  }
+  ForceArgs<TS, NC> fz = fa;
+  fz.F = aview<TS>(Fout, g);
+  {
+    KTimer t(c, "force_from_z");
+    hipLaunchKernelGGL((k_force_from_Z<TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fz, mview<TS>(ws->Z, g));
+  }
+  CHECK_LAUNCH(c);
+  if (out == OUT_PADD) return mom_update(c, g, std::is_same<TS, double>::value ? MAG_F64 : MAG_F32, P, Fout, eps);
+  return MAG_OK;
+}
+
+// force_gradient_integrator (field-hmc.h:525-567). The auxiliary field is
+// written directly by the first force of each half step (no copy), and the
+// second force of each half step updates the momenta in its epilogue.
+template <class TS, int NC>
+int Ops<TS, NC>::fg(mag_ctx c, const Geom& g, void* U, void* P, const mag_action& act, int steps, double dt,
+                    Workspace* ws) {
+  int r;
+  if ((r = ws->need_md())) return r;
+  const double alpha = (3. - std::sqrt(3.)) * dt / 6.;
+  const double beta = dt / std::sqrt(3.);
+  const double gamma = (2. - std::sqrt(3.)) * dt * dt / 12.;
+  if ((r = link_update(c, g, U, U, P, alpha, act.mag))) return r;
+  for (int i = 0; i < steps; ++i) {
+    for (int half = 0; half < 2; ++half) {
+      if ((r = force(c, g, U, act, OUT_UAUX, nullptr, nullptr, ws->Uaux, gamma, ws))) return r;
+      if ((r = force(c, g, ws->Uaux, act, OUT_PADD, nullptr, P, nullptr, 0.5 * dt, ws))) return r;
+      if (half == 0 && (r = link_update(c, g, U, U, P, beta, act.mag))) return r;
+    }
+    const double a = (i < steps - 1) ? 2. * alpha : alpha;
+    if ((r = link_update(c, g, U, U, P, a, act.mag))) return r;
+  }
+  return MAG_OK;
+}
+
+// leap_frog_integrator (field-hmc.h:569-587).
+template <class TS, int NC>
+int Ops<TS, NC>::leapfrog(mag_ctx c, const Geom& g, void* U, void* P, const mag_action& act, int steps, double dt,
+                          Workspace* ws) {
+  int r;
+  if ((r = ws->need_md())) return r;
+  if ((r = link_update(c, g, U, U, P, dt / 2., act.mag))) return r;
+  for (int i = 0; i < steps; ++i) {
+    if ((r = force(c, g, U, act, OUT_PADD, nullptr, P, nullptr, dt, ws))) return r;
+    const double a = (i < steps - 1) ? dt : dt / 2.;
+    if ((r = link_update(c, g, U, U, P, a, act.mag))) return r;
+  }
+  return MAG_OK;
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::action_sums(mag_ctx c, const Geom& g, void* U, bool rect, int slot) {
+  const int nb = (g.V + BLOCK - 1) / BLOCK;
+  int r;
+  if ((r = ensure_part(c, 2 * (size_t)nb))) return r;
+  {
+    KTimer t(c, "action");
+    hipLaunchKernelGGL((k_action_partial<TS, NC>), dim3(nb), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), g,
+                       c->d_part, rect ? 1 : 0);
+  }
+  CHECK_LAUNCH(c);
+  if ((r = final_reduce(c, c->d_part, nb, 2, 0, slot))) return r;
+  return final_reduce(c, c->d_part + 1, nb, 2, 0, slot + 1);
+}
+
+// The Hamiltonian's action on the smeared field: smearing in FP64 from the
+// stored links (FP32 or FP64), then the FP64 plaquette/rectangle sums.
+template <class TS, int NC>
+int Ops<TS, NC>::smeared_action_sums(mag_ctx c, const Geom& g, void* U, double rho, int n, bool rect, int slot) {
+  if (n <= 0) return action_sums(c, g, U, rect, slot);
+  Workspace* wsd = workspace(c, g, MAG_F64, 18);
+  int r;
+  if ((r = wsd->need_h(n))) return r;
+  for (int k = 0; k < n; ++k) {
+    void* dst = wsd->Uh[k & 1];
+    KTimer t(c, "stout_fwd_h");
+    if (k == 0) {
+      StoutFwdArgs<TS, NC, double, 18> fw{g, lview<TS, NC>(U, g), aview<double>(nullptr, g),
+                                          mview<double>(nullptr, g), lview<double, 18>(dst, g), rho, 0};
+      hipLaunchKernelGGL((k_stout_fwd<TS, NC, double, 18, double>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
+    } else {
+      StoutFwdArgs<double, 18, double, 18> fw{g, lview<double, 18>(wsd->Uh[(k - 1) & 1], g),
+                                              aview<double>(nullptr, g), mview<double>(nullptr, g),
+                                              lview<double, 18>(dst, g), rho, 0};
+      hipLaunchKernelGGL((k_stout_fwd<double, 18, double, 18, double>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
+    }
+    CHECK_LAUNCH(c);
+  }
+  void* top = wsd->Uh[(n - 1) & 1];
+  const int nb = (g.V + BLOCK - 1) / BLOCK;
+  if ((r = ensure_part(c, 2 * (size_t)nb))) return r;
+  {
+    KTimer t(c, "action");
+    hipLaunchKernelGGL((k_action_partial<double, 18>), dim3(nb), dim3(BLOCK), 0, c->stream,
+                       lview<double, 18>(top, g), g, c->d_part, rect ? 1 : 0);
+  }
+  CHECK_LAUNCH(c);
+  if ((r = final_reduce(c, c->d_part, nb, 2, 0, slot))) return r;
+  return final_reduce(c, c->d_part + 1, nb, 2, 0, slot + 1);
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::smear(mag_ctx c, const Geom& g, void* Uout, void* U, double rho, int n, Workspace* ws) {
+  int r;
+  if ((r = ws->need_md())) return r;
+  void* src = U;
+  for (int k = 0; k < n; ++k) {
+    void* dst = (k == n - 1) ? Uout : ((src == ws->Uaux) ? Uout : ws->Uaux);
+    if (dst == src) dst = (src == Uout) ? ws->Uaux : Uout;
+    StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(src, g), aview<TS>(nullptr, g), mview<TS>(nullptr, g),
+                                    lview<TS, NC>(dst, g), rho, 0};
+    hipLaunchKernelGGL((k_stout_fwd<TS, NC, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
+    CHECK_LAUNCH(c);
+    src = dst;
+  }
+  if (src != Uout)
+    HIPCHK(c, hipMemcpyAsync(Uout, src, (size_t)g.V * 4 * NC * sizeof(TS), hipMemcpyDeviceToDevice, c->stream));
+  return MAG_OK;
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::reunitarize(mag_ctx c, const Geom& g, void* U, int slot) {
+  const int nb = (g.V + BLOCK - 1) / BLOCK;
+  int r;
+  if ((r = ensure_part(c, 4 * (size_t)nb))) return r;
+  {
+    KTimer t(c, "reunitarize");
+    hipLaunchKernelGGL((k_reunitarize<TS, NC>), grid_links(g), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), g.V,
+                       c->d_part);
+  }
+  CHECK_LAUNCH(c);
+  return final_reduce(c, c->d_part, 4 * nb, 1, 1, slot);
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::max_diff(mag_ctx c, const Geom& g, void* A, void* B, int slot) {
+  const int nb = (g.V + BLOCK - 1) / BLOCK;
+  int r;
+  if ((r = ensure_part(c, 4 * (size_t)nb))) return r;
+  hipLaunchKernelGGL((k_max_diff<TS, NC>), grid_links(g), dim3(BLOCK), 0, c->stream, lview<TS, NC>(A, g),
+                     lview<TS, NC>(B, g), g.V, c->d_part);
+  CHECK_LAUNCH(c);
+  return final_reduce(c, c->d_part, 4 * nb, 1, 1, slot);
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::haar(mag_ctx c, const Geom& g, void* U, uint64_t seed) {
+  hipLaunchKernelGGL((k_haar<TS, NC>), grid_links(g), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), g.V, seed);
+  CHECK_LAUNCH(c);
+  return MAG_OK;
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::unit(mag_ctx c, const Geom& g, void* U) {
+  hipLaunchKernelGGL((k_unit<TS, NC>), grid_links(g), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), g.V);
+  CHECK_LAUNCH(c);
+  return MAG_OK;
+}
+
+template <class TS, int NC>
+int Ops<TS, NC>::wilson_loop(mag_ctx c, const Geom& g, void* U, const int r3[3], int T, int slot, int* nperm) {
+  WloopArgs a{};
+  a.g = g;
+  a.T = T;
+  int rl[3] = {r3[0], r3[1], r3[2]};
+  std::sort(rl, rl + 3);
+  a.nperm = 0;
+  do {
+    for (int i = 0; i < 3; ++i) a.r[a.nperm][i] = rl[i];
+    a.nperm++;
+  } while (std::next_permutation(rl, rl + 3));
+  const int nb = (g.V + BLOCK - 1) / BLOCK;
+  int r;
+  if ((r = ensure_part(c, (size_t)nb))) return r;
+  {
+    KTimer t(c, "wilson_loop");
+    hipLaunchKernelGGL((k_wilson_loop<TS, NC>), dim3(nb), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), a, c->d_part);
+  }
+  CHECK_LAUNCH(c);
+  *nperm = a.nperm;
+  return final_reduce(c, c->d_part, nb, 1, 0, slot);
+}
+
+}  // namespace mag

@@ -1,0 +1,310 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of the FP64 CPU restatement of the reference constrained-HMC
+path (oracle/oracle.cpp). Only tests/, ``__graft_entry__.smoke()`` and the
+benchmark's CPU-baseline leg import this module; the shipped library never
+does. Fields are numpy float64 arrays of shape (V, 4, 3, 3) complex128 or the
+equivalent (V, 4, 18) reals, in the reference's qlat layout.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class Action(C.Structure):
+    _fields_ = [("type", C.c_int), ("beta", C.c_double), ("c1", C.c_double), ("mag", C.c_int),
+                ("n_stout", C.c_int), ("rho", C.c_double)]
+
+
+class Hmc(C.Structure):
+    _fields_ = [("trajectory_length", C.c_int), ("dt", C.c_double), ("integrator", C.c_int),
+                ("num_forced_accept", C.c_int), ("accept_mode", C.c_int), ("demon_emax", C.c_double),
+                ("seed", C.c_uint64), ("reunitarize", C.c_int)]
+
+
+class Traj(C.Structure):
+    _fields_ = [("h_old", C.c_double), ("h_new", C.c_double), ("dh", C.c_double), ("die_roll", C.c_double),
+                ("accept", C.c_int), ("avg_plaq", C.c_double), ("s_gauge", C.c_double),
+                ("demon_e", C.c_double * 2), ("reunit_dev", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_U32 = C.POINTER(C.c_uint32)
+
+
+def _declare(l):
+    sig = {
+        "orc_philox4x32_10": (None, [_U32, _U32, _U32]),
+        "orc_generators": (None, [_D]),
+        "orc_expiQ": (None, [_D, _D, C.c_long]),
+        "orc_DexpiQ": (None, [_D, _D, _D, C.c_long]),
+        "orc_compute_Lambda": (None, [_D, _D, _D, _D, C.c_long]),
+        "orc_haar_links": (None, [_I, C.c_uint64, _D]),
+        "orc_unit_links": (None, [_I, _D]),
+        "orc_init_momentum": (None, [_I, C.c_uint64, C.c_uint64, C.c_double, _D]),
+        "orc_uniform": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint64]),
+        "orc_path_product": (None, [_I, _D, _I, _I, C.c_int, _D]),
+        "orc_staple_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
+        "orc_rectangular_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
+        "orc_chair_staple_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
+        "orc_twist_staple_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
+        "orc_is_constrained": (C.c_int, [_I, C.c_int, C.c_int]),
+        "orc_total_plaq": (C.c_double, [_I, _D]),
+        "orc_total_rectangular": (C.c_double, [_I, _D]),
+        "orc_avg_plaquette": (C.c_double, [_I, _D]),
+        "orc_kinetic": (C.c_double, [_I, _D, C.c_int]),
+        "orc_gauge_action": (C.c_double, [_I, _D, C.POINTER(Action)]),
+        "orc_hamiltonian": (C.c_double, [_I, _D, _D, C.POINTER(Action), _D]),
+        "orc_reunitarize": (C.c_double, [_I, _D]),
+        "orc_get_force": (None, [_I, _D, _D, C.POINTER(Action)]),
+        "orc_evolve_gauge_field": (None, [_I, _D, _D, C.c_double, C.c_int]),
+        "orc_evolve_momentum": (None, [_I, _D, _D, C.c_double, C.c_double]),
+        "orc_stout_smear": (None, [_I, _D, C.c_double, C.c_int, _D]),
+        "orc_stout_Q": (None, [_I, _D, C.c_double, _D]),
+        "orc_force_gradient_integrator": (C.c_double, [_I, _D, _D, C.POINTER(Action), C.c_int, C.c_double, C.c_int]),
+        "orc_leap_frog_integrator": (None, [_I, _D, _D, C.POINTER(Action), C.c_int, C.c_double]),
+        "orc_avg_wilson_loop": (C.c_double, [_I, _D, _I, C.c_int]),
+        "orc_run_hmc": (C.c_int, [_I, _D, C.POINTER(Action), C.POINTER(Hmc), C.c_int, C.c_int, _D,
+                                  C.POINTER(Traj)]),
+        "orc_demon_microcanonical": (None, [_I, _D, C.c_int, C.c_double, C.c_uint64, _D,
+                                            C.POINTER(C.c_long)]),
+        "orc_num_threads": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = args
+
+
+# ----------------------------------------------------------------- helpers
+def _dims(L):
+    return (C.c_int * 4)(*[int(v) for v in L])
+
+
+def _dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def as_real(m):
+    """complex (..., 3, 3) -> float64 (..., 18) view-compatible copy."""
+    m = np.ascontiguousarray(m, dtype=np.complex128)
+    return m.view(np.float64).reshape(m.shape[:-2] + (18,))
+
+
+def as_cplx(r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return r.view(np.complex128).reshape(r.shape[:-1] + (3, 3))
+
+
+def volume(L):
+    return int(np.prod(L))
+
+
+def action(type=0, beta=6.0, c1=0.0, mag=0, n_stout=0, rho=0.1):
+    return Action(type, beta, c1, mag, n_stout, rho)
+
+
+def hmc(trajectory_length=8, dt=0.125, integrator=0, num_forced_accept=0, accept_mode=0, demon_emax=0.0,
+        seed=1234, reunitarize=1):
+    return Hmc(trajectory_length, dt, integrator, num_forced_accept, accept_mode, demon_emax, seed, reunitarize)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def generators():
+    out = np.zeros((8, 18))
+    lib().orc_generators(_dptr(out))
+    return as_cplx(out)
+
+
+def _batch(fn, *mats):
+    arrs = [as_real(np.asarray(m)).reshape(-1, 18) for m in mats]
+    n = arrs[0].shape[0]
+    out = np.zeros((n, 18))
+    fn(*[_dptr(a) for a in arrs], _dptr(out), n)
+    return as_cplx(out).reshape(np.asarray(mats[0]).shape)
+
+
+def expiQ(Q):
+    return _batch(lib().orc_expiQ, Q)
+
+
+def DexpiQ(Q, DQ):
+    return _batch(lib().orc_DexpiQ, Q, DQ)
+
+
+def compute_Lambda(Q, S, U):
+    return _batch(lib().orc_compute_Lambda, Q, S, U)
+
+
+def haar_links(L, seed):
+    U = np.zeros((volume(L), 4, 18))
+    lib().orc_haar_links(_dims(L), seed, _dptr(U))
+    return U
+
+
+def unit_links(L):
+    U = np.zeros((volume(L), 4, 18))
+    lib().orc_unit_links(_dims(L), _dptr(U))
+    return U
+
+
+def init_momentum(L, seed, traj, m=1.0):
+    P = np.zeros((volume(L), 4, 18))
+    lib().orc_init_momentum(_dims(L), seed, traj, m, _dptr(P))
+    return P
+
+
+def uniform(seed, stream, index):
+    return lib().orc_uniform(seed, stream, index)
+
+
+def path_product(L, U, x, dirs):
+    out = np.zeros(18)
+    xs = (C.c_int * 4)(*x)
+    ds = (C.c_int * len(dirs))(*dirs)
+    lib().orc_path_product(_dims(L), _dptr(U), xs, ds, len(dirs), _dptr(out))
+    return as_cplx(out)
+
+
+def _site(fn, L, U, site, mu):
+    out = np.zeros(18)
+    fn(_dims(L), _dptr(U), site, mu, _dptr(out))
+    return as_cplx(out)
+
+
+def staple_dagger(L, U, site, mu):
+    return _site(lib().orc_staple_dagger, L, U, site, mu)
+
+
+def rectangular_dagger(L, U, site, mu):
+    return _site(lib().orc_rectangular_dagger, L, U, site, mu)
+
+
+def chair_staple_dagger(L, U, site, mu):
+    return _site(lib().orc_chair_staple_dagger, L, U, site, mu)
+
+
+def twist_staple_dagger(L, U, site, mu):
+    return _site(lib().orc_twist_staple_dagger, L, U, site, mu)
+
+
+def is_constrained(x, mu, mag):
+    return lib().orc_is_constrained((C.c_int * 4)(*x), mu, mag)
+
+
+def total_plaq(L, U):
+    return lib().orc_total_plaq(_dims(L), _dptr(U))
+
+
+def total_rectangular(L, U):
+    return lib().orc_total_rectangular(_dims(L), _dptr(U))
+
+
+def avg_plaquette(L, U):
+    return lib().orc_avg_plaquette(_dims(L), _dptr(U))
+
+
+def kinetic(L, P, mag=0):
+    return lib().orc_kinetic(_dims(L), _dptr(P), mag)
+
+
+def gauge_action(L, U, act):
+    return lib().orc_gauge_action(_dims(L), _dptr(U), C.byref(act))
+
+
+def hamiltonian(L, U, P, act):
+    parts = np.zeros(2)
+    h = lib().orc_hamiltonian(_dims(L), _dptr(U), _dptr(P), C.byref(act), _dptr(parts))
+    return h, parts
+
+
+def reunitarize(L, U):
+    return lib().orc_reunitarize(_dims(L), _dptr(U))
+
+
+def get_force(L, U, act):
+    F = np.zeros_like(U)
+    lib().orc_get_force(_dims(L), _dptr(U), _dptr(F), C.byref(act))
+    return F
+
+
+def evolve_gauge_field(L, U, P, dt, mag=0):
+    lib().orc_evolve_gauge_field(_dims(L), _dptr(U), _dptr(P), dt, mag)
+
+
+def evolve_momentum(L, P, F, dt, m=1.0):
+    lib().orc_evolve_momentum(_dims(L), _dptr(P), _dptr(F), dt, m)
+
+
+def stout_smear(L, U, rho, nlev):
+    out = np.zeros_like(U)
+    lib().orc_stout_smear(_dims(L), _dptr(U), rho, nlev, _dptr(out))
+    return out
+
+
+def stout_Q(L, U, rho):
+    out = np.zeros_like(U)
+    lib().orc_stout_Q(_dims(L), _dptr(U), rho, _dptr(out))
+    return out
+
+
+def force_gradient_integrator(L, U, P, act, steps, dt, reunit=True):
+    return lib().orc_force_gradient_integrator(_dims(L), _dptr(U), _dptr(P), C.byref(act), steps, dt,
+                                               1 if reunit else 0)
+
+
+def leap_frog_integrator(L, U, P, act, steps, dt):
+    lib().orc_leap_frog_integrator(_dims(L), _dptr(U), _dptr(P), C.byref(act), steps, dt)
+
+
+def avg_wilson_loop(L, U, r, T):
+    return lib().orc_avg_wilson_loop(_dims(L), _dptr(U), (C.c_int * 3)(*r), T)
+
+
+def run_hmc(L, U, act, h, first_traj, ntraj, demon_state=None):
+    out = (Traj * ntraj)()
+    ds = np.zeros(4) if demon_state is None else demon_state
+    nacc = lib().orc_run_hmc(_dims(L), _dptr(U), C.byref(act), C.byref(h), first_traj, ntraj, _dptr(ds), out)
+    return nacc, [out[i] for i in range(ntraj)], ds
+
+
+def demon_microcanonical(L, U, nsweeps, E_max, seed, E=None):
+    E = np.zeros(4) if E is None else E
+    ar = (C.c_long * 2)()
+    lib().orc_demon_microcanonical(_dims(L), _dptr(U), nsweeps, E_max, seed, _dptr(E), ar)
+    return E, (ar[0], ar[1])
+
+
+def num_threads():
+    return lib().orc_num_threads()

@@ -1,0 +1,284 @@
+"""GPU parity: the gfx950 HIP library (through its C ABI) against the FP64
+CPU oracle on the same seeded inputs.
+
+Tolerances (FP64 path): per-matrix functions 1e-14 absolute; forces and
+smeared fields 1e-12 relative (max-norm of the difference over max-norm of
+the field); plaquette, rectangle, action and Hamiltonian 1e-13 relative;
+identical accept/reject and demon sequences. FP32 molecular dynamics: the
+errors against the FP64 oracle are asserted at the bounds stated per test.
+"""
+import numpy as np
+import pytest
+
+import mag_amd as M
+from mag_amd import F32, F64, LINK12, LINK18, ALG8
+
+pytestmark = pytest.mark.gpu
+
+L4 = (4, 4, 4, 4)
+L8 = (8, 8, 8, 8)
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def links(ctx, L, host, prec=F64):
+    kind = LINK18 if prec == F64 else LINK12
+    return M.Field(ctx, L, kind, prec).upload(host)
+
+
+def alg(ctx, L, host=None, prec=F64):
+    f = M.Field(ctx, L, ALG8, prec)
+    if host is not None:
+        f.upload(host)
+    return f
+
+
+def herm(orc, rng, scale, n):
+    T = orc.generators()
+    return np.einsum("na,aij->nij", rng.normal(size=(n, 8)) * scale, T)
+
+
+# ------------------------------------------------------------------ per link
+@pytest.mark.parametrize("scale", [1e-5, 1e-2, 0.3, 1.5])
+def test_expiQ_batch(ctx, orc, scale):
+    rng = np.random.default_rng(1)
+    Q = herm(orc, rng, scale, 4096)
+    ref = orc.expiQ(Q)
+    assert np.abs(M.expiQ(ctx, Q, F64) - ref).max() < 1e-14 * max(1, scale)
+    assert np.abs(M.expiQ(ctx, Q, F32) - ref).max() < 2e-6
+
+
+def test_compute_Lambda_batch(ctx, orc):
+    rng = np.random.default_rng(2)
+    Q = herm(orc, rng, 0.4, 2048)
+    S = rng.normal(size=(2048, 3, 3)) + 1j * rng.normal(size=(2048, 3, 3))
+    U = orc.as_cplx(orc.haar_links(L4, 3)).reshape(-1, 3, 3)[:2048]
+    ref = orc.compute_Lambda(Q, S, U)
+    assert rel(M.compute_Lambda(ctx, Q, S, U, F64), ref) < 1e-12
+    assert rel(M.compute_Lambda(ctx, Q, S, U, F32), ref) < 5e-5
+
+
+# ------------------------------------------------------------------ fields
+def test_upload_download_roundtrip(ctx, orc):
+    U = orc.haar_links(L4, 4)
+    assert np.array_equal(links(ctx, L4, U).download(), U)
+    d32 = links(ctx, L4, U, F32).download()
+    assert np.abs(d32 - U).max() < 5e-7  # 12-real FP32 storage, row 2 rebuilt
+    P = orc.init_momentum(L4, 4, 0)
+    assert np.abs(alg(ctx, L4, P).download() - P).max() < 1e-15
+
+
+def test_haar_and_momentum_streams_match_oracle(ctx, orc):
+    U = M.Field(ctx, L8, LINK18, F64).haar(1234)
+    assert np.abs(U.download() - orc.haar_links(L8, 1234)).max() < 1e-13
+    P = alg(ctx, L8)
+    M.init_momentum(P, 1234, 7)
+    assert np.abs(P.download() - orc.init_momentum(L8, 1234, 7)).max() < 1e-13
+
+
+def test_reductions(ctx, orc):
+    U = orc.haar_links(L8, 1234)
+    d = links(ctx, L8, U)
+    assert rel(M.total_plaq(d), orc.total_plaq(L8, U)) < 1e-13
+    assert rel(M.total_rectangular(d), orc.total_rectangular(L8, U)) < 1e-13
+    assert rel(M.avg_plaquette(d), orc.avg_plaquette(L8, U)) < 1e-13
+    for r, T in [((1, 0, 0), 1), ((2, 1, 0), 2), ((1, 1, 1), 3)]:
+        assert abs(M.avg_wilson_loop(d, r, T) - orc.avg_wilson_loop(L8, U, r, T)) < 1e-13
+    unit = M.Field(ctx, L8, LINK18, F64).unit()
+    assert M.total_plaq(unit) == 18 * 8 ** 4
+    assert M.avg_wilson_loop(unit, (4, 0, 0), 4) == 3.0
+
+
+ACTIONS = {
+    "wilson": dict(type=0, beta=6.0),
+    "iwasaki": dict(type=1, beta=2.6, c1=-0.331),
+    "dbw2": dict(type=1, beta=1.04, c1=-1.4069),
+    "iwasaki_mag2": dict(type=1, beta=2.6, c1=-0.331, mag=2),
+    "wilson_stout1": dict(type=0, beta=5.8, n_stout=1, rho=0.1),
+    "iwasaki_stout3": dict(type=1, beta=2.6, c1=-0.331, n_stout=3, rho=0.1),
+    "wilson_stout2_mag2": dict(type=0, beta=6.0, mag=2, n_stout=2, rho=0.12),
+}
+
+
+@pytest.mark.parametrize("name", list(ACTIONS))
+def test_force_fp64(ctx, orc, name):
+    a = orc.action(**ACTIONS[name])
+    L = L8 if a.n_stout == 0 else L4
+    U = orc.haar_links(L, 17)
+    ref = orc.get_force(L, U, a)
+    F = alg(ctx, L)
+    M.get_force(F, links(ctx, L, U), M._lib.Action(a.type, a.beta, a.c1, a.mag, a.n_stout, a.rho))
+    assert rel(F.download(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["wilson", "iwasaki", "iwasaki_stout3"])
+def test_force_fp32_error(ctx, orc, name):
+    """FP32 molecular-dynamics force against the FP64 oracle on the same (FP32-rounded) links."""
+    a = orc.action(**ACTIONS[name])
+    L = L8 if a.n_stout == 0 else L4
+    U32 = links(ctx, L, orc.haar_links(L, 19), F32)
+    U = U32.download()  # the FP32 links, exactly, as FP64
+    ref = orc.get_force(L, U, a)
+    F = alg(ctx, L, prec=F32)
+    M.get_force(F, U32, M._lib.Action(a.type, a.beta, a.c1, a.mag, a.n_stout, a.rho))
+    err = rel(F.download(), ref)
+    assert err < 2e-5, err
+
+
+@pytest.mark.parametrize("mag", [0, 2])
+def test_evolve_gauge_and_momentum(ctx, orc, mag):
+    U = orc.haar_links(L8, 23)
+    P = orc.init_momentum(L8, 23, 0)
+    dU, dP = links(ctx, L8, U), alg(ctx, L8, P)
+    M.evolve_gauge_field(dU, dP, 0.137, M._lib.Action(0, 6.0, 0.0, mag, 0, 0.0))
+    Ur = U.copy()
+    orc.evolve_gauge_field(L8, Ur, P, 0.137, mag)
+    assert np.abs(dU.download() - Ur).max() < 1e-13
+    F = orc.get_force(L8, U, orc.action(1, 2.6, -0.331))
+    dF = alg(ctx, L8, F)
+    M.evolve_momentum(dP, dF, 0.25)
+    Pr = P.copy()
+    orc.evolve_momentum(L8, Pr, F, 0.25)
+    assert np.abs(dP.download() - Pr).max() < 1e-13
+
+
+def test_stout_smear_fp64(ctx, orc):
+    U = orc.haar_links(L4, 29)
+    out = M.Field(ctx, L4, LINK18, F64)
+    M.stout_smear(out, links(ctx, L4, U), 0.1, 3)
+    assert rel(out.download(), orc.stout_smear(L4, U, 0.1, 3)) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["wilson", "iwasaki_stout3", "iwasaki_mag2"])
+def test_hamiltonian(ctx, orc, name):
+    a = orc.action(**ACTIONS[name])
+    L = L4 if a.n_stout else L8
+    U, P = orc.haar_links(L, 31), orc.init_momentum(L, 31, 0)
+    h_ref, parts_ref = orc.hamiltonian(L, U, P, a)
+    h, parts = M.get_hamiltonian(links(ctx, L, U), alg(ctx, L, P),
+                                 M._lib.Action(a.type, a.beta, a.c1, a.mag, a.n_stout, a.rho))
+    assert rel(h, h_ref) < 1e-13
+    assert rel(parts, parts_ref) < 1e-13
+
+
+def test_fg_trajectory_c1(ctx, orc):
+    """Config C1: 8^4 Wilson beta=6.0, Haar links seed 1234, FG tau=1 with 8 steps."""
+    a = orc.action(0, 6.0)
+    U, P = orc.haar_links(L8, 1234), orc.init_momentum(L8, 1234, 0)
+    h0, _ = orc.hamiltonian(L8, U, P, a)
+    Ur, Pr = U.copy(), P.copy()
+    dev_ref = orc.force_gradient_integrator(L8, Ur, Pr, a, 8, 0.125, reunit=True)
+    h1, _ = orc.hamiltonian(L8, Ur, Pr, a)
+    dU, dP = links(ctx, L8, U), alg(ctx, L8, P)
+    arg = M.Arg_chmc(trajectory_length=8, dt=0.125, beta=6.0)
+    dev = M.force_gradient_integrator(dU, dP, arg)
+    g0, _ = M.get_hamiltonian(dU, dP, arg)
+    assert rel(dU.download(), Ur) < 1e-10
+    assert rel(dP.download(), Pr) < 1e-10
+    assert abs(g0 - h1) < 1e-9 * abs(h1)
+    assert abs((g0 - h0) - (h1 - h0)) < 1e-6 * max(1.0, abs(h1 - h0))
+    assert dev < 1e-12 and dev_ref < 1e-12
+
+
+def test_fg_trajectory_stout_fp64(ctx, orc):
+    a = orc.action(1, 2.6, -0.331, n_stout=2, rho=0.1)
+    U, P = orc.haar_links(L4, 37), orc.init_momentum(L4, 37, 0)
+    Ur, Pr = U.copy(), P.copy()
+    orc.force_gradient_integrator(L4, Ur, Pr, a, 3, 0.1, reunit=False)
+    dU, dP = links(ctx, L4, U), alg(ctx, L4, P)
+    arg = M.Arg_chmc(trajectory_length=3, dt=0.1, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=2, rho=0.1)
+    M.force_gradient_integrator(dU, dP, arg, reunit=False)
+    assert rel(dU.download(), Ur) < 1e-10
+    assert rel(dP.download(), Pr) < 1e-10
+
+
+def _chain_vs_oracle(ctx, orc, L, act_kw, h_kw, ntraj, mode=0, emax=0.0, links_seed=1234):
+    a = orc.action(**act_kw)
+    U = orc.haar_links(L, links_seed)
+    h = orc.hmc(**h_kw, accept_mode=mode, demon_emax=emax)
+    ds = None
+    if mode:
+        # the chain's window is relative to the initial top-level action sums
+        top = U if a.n_stout == 0 else orc.stout_smear(L, U, a.rho, a.n_stout)
+        ds = np.array([-orc.total_plaq(L, top), -orc.total_rectangular(L, top), 0.0, 0.0])
+    Ur = U.copy()
+    nacc, ref, _ = orc.run_hmc(L, Ur, a, h, 0, ntraj, ds)
+    arg = M.Arg_chmc(trajectory_length=h.trajectory_length, dt=h.dt, beta=a.beta, gauge=M.Gauge(a.type, a.c1),
+                     mag=a.mag, n_stout=a.n_stout, rho=a.rho, num_forced_accept_step=h.num_forced_accept)
+    ch = M.Chain(ctx, L, arg, seed=h.seed, precision=F64, accept_mode=mode, demon_emax=emax)
+    ch.set_links(U)
+    res = ch.run(ntraj)
+    return ref, res, Ur, ch
+
+
+def test_accept_sequence_20_trajectories(ctx, orc):
+    ref, res, Ur, ch = _chain_vs_oracle(ctx, orc, L4, dict(type=0, beta=5.7),
+                                        dict(trajectory_length=6, dt=1 / 6, num_forced_accept=3, seed=99), 20)
+    assert [r.accept for r in res] == [r.accept for r in ref]
+    assert 0 < sum(r.accept for r in ref[3:]) < 17  # both outcomes occur
+    for r, q in zip(res, ref):
+        assert r.die_roll == q.die_roll
+        assert abs(r.h_old - q.h_old) < 1e-12 * abs(q.h_old)
+        assert abs(r.dh - q.dh) < 1e-8 * max(1, abs(q.dh))
+        assert abs(r.avg_plaq - q.avg_plaq) < 1e-12
+    assert rel(ch.get_links(), Ur) < 1e-9
+
+
+def test_demon_window_sequence(ctx, orc):
+    """Constrained Metropolis + demon window (accept mode 2) on an Iwasaki + stout chain."""
+    ref, res, Ur, ch = _chain_vs_oracle(ctx, orc, L4, dict(type=1, beta=2.6, c1=-0.331, n_stout=1, rho=0.1),
+                                        dict(trajectory_length=4, dt=0.25, num_forced_accept=0, seed=3), 8,
+                                        mode=2, emax=6.0)
+    assert [r.accept for r in res] == [r.accept for r in ref]
+    for r, q in zip(res, ref):
+        assert abs(r.demon_e[0] - q.demon_e[0]) < 1e-9 and abs(r.demon_e[1] - q.demon_e[1]) < 1e-9
+
+
+def test_reversibility_fp64(ctx, orc):
+    arg = M.Arg_chmc(trajectory_length=8, dt=0.125, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=1, rho=0.1)
+    ch = M.Chain(ctx, L4, arg, seed=5)
+    ch.set_links(orc.haar_links(L4, 5))
+    dev, dh = ch.reversibility()
+    assert dev < 1e-10
+    assert abs(dh) < 1e-8
+
+
+def test_fp32_chain_errors(ctx, orc):
+    """FP32-MD / FP64-dH chain against the FP64 chain from the same start and seed."""
+    arg = M.Arg_chmc(trajectory_length=8, dt=0.125, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=1, rho=0.1,
+                     num_forced_accept_step=2)
+    U = orc.haar_links(L8, 43)
+    c64 = M.Chain(ctx, L8, arg, seed=43, precision=F64)
+    c32 = M.Chain(ctx, L8, arg, seed=43, precision=F32)
+    c64.set_links(U)
+    c32.set_links(U)
+    r64, r32 = c64.run(3), c32.run(3)
+    for a, b in zip(r64, r32):
+        assert abs(a.h_old - b.h_old) < 1e-5 * abs(a.h_old)
+        assert abs(a.dh - b.dh) < 5e-2 * max(1.0, abs(a.dh))
+
+
+def test_demon_microcanonical_sequence(ctx, orc):
+    U = orc.haar_links(L4, 111)
+    Ur = U.copy()
+    E_ref, ar_ref = orc.demon_microcanonical(L4, Ur, 1, 40.0, 5)
+    d = links(ctx, L4, U)
+    E, ar = M.demon_microcanonical(d, 1, 40.0, 5)
+    assert ar == ar_ref
+    assert np.abs(E - E_ref).max() < 1e-9
+    assert np.abs(d.download() - Ur).max() < 1e-12
+
+
+def test_bitwise_determinism(ctx, orc):
+    arg = M.Arg_chmc(trajectory_length=4, dt=0.25, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=2, rho=0.1)
+    U = orc.haar_links(L4, 47)
+    outs = []
+    for _ in range(2):
+        ch = M.Chain(ctx, L4, arg, seed=47, precision=F32)
+        ch.set_links(U)
+        r = ch.run(2)
+        outs.append((ch.get_links(), [x.h_new for x in r]))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
