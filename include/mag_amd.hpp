@@ -1,0 +1,203 @@
+// mag_amd.hpp — C++ host API with the reference's entry-point names
+// (hummingtree/mag field-hmc.h, field-matrix.h, field-wloop.h,
+// field-demon.h), as thin inline wrappers over the C ABI of mag_amd.h.
+// A caller of the reference replaces qlat::Field<cps::Matrix> by
+// mag::Field and keeps its call sequence:
+//
+//   mag::Context ctx(0);
+//   mag::Field U(ctx, {32, 32, 32, 64}, MAG_LINK18, MAG_F64), P(...ALG8...);
+//   mag::Arg_chmc arg; arg.trajectory_length = 12; arg.dt = 1. / 12; ...
+//   mag::init_momentum(P, seed, traj);
+//   std::vector<double> part;
+//   double H0 = mag::get_hamiltonian(U, P, arg, part);
+//   mag::force_gradient_integrator(U, P, arg);
+//
+// Errors are thrown as mag::Error (status code + message); the reference
+// aborts through assert instead.
+#pragma once
+#include <array>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mag_amd.h"
+
+namespace mag {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int code, mag_ctx ctx) {
+  if (code != MAG_OK) throw Error(code, std::string("mag_amd: ") + (ctx ? mag_last_error(ctx) : "error"));
+}
+
+class Context {
+ public:
+  explicit Context(int device = 0) { check(mag_ctx_create(device, &h_), nullptr); }
+  ~Context() {
+    if (h_) mag_ctx_destroy(h_);
+  }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  mag_ctx get() const { return h_; }
+  void synchronize() const { check(mag_ctx_synchronize(h_), h_); }
+
+ private:
+  mag_ctx h_ = nullptr;
+};
+
+class Field {
+ public:
+  Field(const Context& ctx, std::array<int, 4> dims, int kind = MAG_LINK18, int prec = MAG_F64)
+      : ctx_(ctx.get()), dims_(dims), kind_(kind) {
+    check(mag_field_create(ctx_, dims.data(), kind, prec, &h_), ctx_);
+  }
+  ~Field() {
+    if (h_) mag_field_destroy(h_);
+  }
+  Field(const Field&) = delete;
+  Field& operator=(const Field&) = delete;
+  // gFieldExt = gField (field-hmc.h:829)
+  void copy_from(const Field& o) { check(mag_field_copy(h_, o.h_), ctx_); }
+  mag_field get() const { return h_; }
+  mag_ctx ctx() const { return ctx_; }
+  long volume() const { return (long)dims_[0] * dims_[1] * dims_[2] * dims_[3]; }
+  // qlat layout [site][mu][3][3][re,im]
+  void upload(const std::vector<double>& aos) { check(mag_field_upload_aos(h_, aos.data()), ctx_); }
+  std::vector<double> download() const {
+    std::vector<double> a((size_t)volume() * 4 * 18);
+    check(mag_field_download_aos(h_, a.data()), ctx_);
+    return a;
+  }
+
+ private:
+  mag_ctx ctx_;
+  std::array<int, 4> dims_;
+  int kind_;
+  mag_field h_ = nullptr;
+};
+
+// Gauge{type, c1} and Arg_chmc (field-hmc.h:294-307), plus smearing.
+struct Gauge {
+  int type = MAG_WILSON;
+  double c1 = 0.;
+};
+struct Arg_chmc {
+  int mag = 0;
+  int trajectory_length = 8;
+  int num_trajectory = 1;
+  int num_step_between_output = 1;
+  int num_forced_accept_step = 0;
+  int num_step_before_output = 0;
+  double beta = 6.0;
+  double dt = 0.125;
+  Gauge gauge;
+  int n_stout = 0;
+  double rho = 0.1;
+  mag_action action() const { return mag_action{gauge.type, beta, gauge.c1, mag, n_stout, rho}; }
+};
+
+// get_force (field-hmc.h:339)
+inline void get_force(Field& F, const Field& U, const Arg_chmc& arg) {
+  mag_action a = arg.action();
+  check(mag_gauge_force(F.get(), U.get(), &a), U.ctx());
+}
+// evolve_momentum (field-hmc.h:427)
+inline void evolve_momentum(Field& P, const Field& F, double dt, const Arg_chmc&, double m = 1.) {
+  check(mag_evolve_momentum(P.get(), F.get(), dt, m), P.ctx());
+}
+// evolve_gauge_field (field-hmc.h:442)
+inline void evolve_gauge_field(Field& U, const Field& P, double dt, const Arg_chmc& arg) {
+  check(mag_evolve_gauge(U.get(), P.get(), dt, arg.mag), U.ctx());
+}
+// force_gradient_integrator (field-hmc.h:525); returns the reunitarize deviation
+inline double force_gradient_integrator(Field& U, Field& P, const Arg_chmc& arg) {
+  mag_action a = arg.action();
+  double dev = 0.;
+  check(mag_fg_integrate(U.get(), P.get(), &a, arg.trajectory_length, arg.dt, 1, &dev), U.ctx());
+  return dev;
+}
+// leap_frog_integrator (field-hmc.h:569)
+inline void leap_frog_integrator(Field& U, Field& P, const Arg_chmc& arg) {
+  mag_action a = arg.action();
+  check(mag_leapfrog_integrate(U.get(), P.get(), &a, arg.trajectory_length, arg.dt), U.ctx());
+}
+// get_hamiltonian (field-hmc.h:589)
+inline double get_hamiltonian(const Field& U, const Field& P, const Arg_chmc& arg, std::vector<double>& part) {
+  mag_action a = arg.action();
+  double parts[2], H = 0.;
+  check(mag_hamiltonian(U.get(), P.get(), &a, parts, &H), U.ctx());
+  part.assign(parts, parts + 2);
+  return H;
+}
+// init_momentum (field-hmc.h:646), Philox stream keyed by (seed, trajectory)
+inline void init_momentum(Field& P, uint64_t seed, uint64_t traj, double m = 1.) {
+  check(mag_init_momentum(P.get(), seed, traj, m), P.ctx());
+}
+// total_plaq / total_rectangular / avg_plaquette (field-matrix.h:299, 354, 387)
+inline double total_plaq(const Field& U) {
+  double v = 0.;
+  check(mag_total_plaq(U.get(), &v), U.ctx());
+  return v;
+}
+inline double total_rectangular(const Field& U) {
+  double v = 0.;
+  check(mag_total_rectangular(U.get(), &v), U.ctx());
+  return v;
+}
+inline double avg_plaquette(const Field& U) {
+  double v = 0.;
+  check(mag_avg_plaquette(U.get(), &v), U.ctx());
+  return v;
+}
+// reunitarize (field-matrix.h:83)
+inline double reunitarize(Field& U) {
+  double v = 0.;
+  check(mag_reunitarize(U.get(), &v), U.ctx());
+  return v;
+}
+// avg_wilson_loop (field-wloop.h:58)
+inline double avg_wilson_loop(const Field& U, const std::array<int, 3>& r, int T) {
+  double v = 0.;
+  check(mag_avg_wilson_loop(U.get(), r.data(), T, &v), U.ctx());
+  return v;
+}
+// demon_microcanonical (field-demon.h:25); E holds the 4 demon energies
+inline std::pair<long, long> demon_microcanonical(Field& U, int nsweeps, double E_max, uint64_t seed,
+                                                  std::array<double, 4>& E) {
+  long ar[2] = {0, 0};
+  check(mag_demon_microcanonical(U.get(), nsweeps, E_max, seed, E.data(), ar), U.ctx());
+  return {ar[0], ar[1]};
+}
+
+// run_chmc (field-hmc.h:770) as a device-resident chain.
+class Chain {
+ public:
+  Chain(const Context& ctx, std::array<int, 4> dims, const Arg_chmc& arg, const mag_hmc_params& p)
+      : ctx_(ctx.get()) {
+    mag_action a = arg.action();
+    check(mag_chain_create(ctx_, dims.data(), &a, &p, &h_), ctx_);
+  }
+  ~Chain() {
+    if (h_) mag_chain_destroy(h_);
+  }
+  Chain(const Chain&) = delete;
+  Chain& operator=(const Chain&) = delete;
+  void set_links(const std::vector<double>& aos) { check(mag_chain_set_links_aos(h_, aos.data()), ctx_); }
+  void haar(uint64_t seed) { check(mag_chain_haar_links(h_, seed), ctx_); }
+  std::vector<mag_traj_result> run(int ntraj) {
+    std::vector<mag_traj_result> r((size_t)ntraj);
+    check(mag_chain_run(h_, ntraj, r.data()), ctx_);
+    return r;
+  }
+  mag_chain get() const { return h_; }
+
+ private:
+  mag_ctx ctx_;
+  mag_chain h_ = nullptr;
+};
+
+}  // namespace mag

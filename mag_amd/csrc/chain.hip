@@ -50,14 +50,14 @@ int traj_kernels(mag_chain c, bool need_rect) {
   if ((r = kinetic_sum(x, g, c->prec, c->P->p, c->act.mag, S_OLD_K))) return r;
   if ((r = O::action_sums(x, g, c->U->p, false, S_THIN_OLD))) return r;
   // backup, integrate (field-hmc.h:814)
-  HIPCHK(x, hipMemcpyAsync(c->Ubak->p, c->U->p, c->U->bytes, hipMemcpyDeviceToDevice, x->stream));
+  if ((r = copy_dev(x, c->Ubak->p, c->U->p, c->U->bytes))) return r;
   if (c->p.integrator == 0) r = O::fg(x, g, c->U->p, c->P->p, c->act, c->p.trajectory_length, c->p.dt, ws);
   else r = O::leapfrog(x, g, c->U->p, c->P->p, c->act, c->p.trajectory_length, c->p.dt, ws);
   if (r) return r;
   if (c->p.reunitarize) {
     if ((r = O::reunitarize(x, g, c->U->p, S_DEV))) return r;
   } else {
-    HIPCHK(x, hipMemsetAsync(x->d_scalar + S_DEV, 0, sizeof(double), x->stream));
+    if ((r = fill_dev(x, x->d_scalar + S_DEV, 0., 1))) return r;
   }
   // H_new (field-hmc.h:816)
   if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, need_rect, S_NEW_P))) return r;

@@ -1,6 +1,7 @@
 // Format-independent operations: final reductions, momenta, conversions and
 // the per-matrix test hooks.
 #include "kernels.hpp"
+#include "launch.hpp"
 #include "runtime.hpp"
 
 namespace mag {
@@ -11,57 +12,48 @@ AlgView<TS> av(void* p, const Geom& g) { return AlgView<TS>{(TS*)p, g.V}; }
 }  // namespace
 
 int final_reduce(mag_ctx c, const double* part, int n, int stride, int op, int slot) {
-  {
-    KTimer t(c, "reduce");
-    if (op)
-      hipLaunchKernelGGL(k_final_reduce<1>, dim3(1), dim3(BLOCK), 0, c->stream, part, n, stride, c->d_scalar + slot);
-    else
-      hipLaunchKernelGGL(k_final_reduce<0>, dim3(1), dim3(BLOCK), 0, c->stream, part, n, stride, c->d_scalar + slot);
-  }
-  CHECK_LAUNCH(c);
-  return MAG_OK;
+  ReduceArgs a{part, n, stride, c->d_scalar + slot};
+  if (op) return launch(c, "reduce", k_final_reduce<1>, k_final_reduce_b<1>, dim3(1), a);
+  return launch(c, "reduce", k_final_reduce<0>, k_final_reduce_b<0>, dim3(1), a);
 }
 
 int kinetic_sum(mag_ctx c, const Geom& g, int prec, void* P, int mag, int slot) {
   const int nb = (g.V + BLOCK - 1) / BLOCK;
   int r;
   if ((r = ensure_part(c, (size_t)nb))) return r;
-  {
-    KTimer t(c, "kinetic");
-    if (prec == MAG_F64)
-      hipLaunchKernelGGL((k_kinetic_partial<double>), dim3(nb), dim3(BLOCK), 0, c->stream, av<double>(P, g), g, mag,
-                         c->d_part);
-    else
-      hipLaunchKernelGGL((k_kinetic_partial<float>), dim3(nb), dim3(BLOCK), 0, c->stream, av<float>(P, g), g, mag,
-                         c->d_part);
+  if (prec == MAG_F64) {
+    KineticArgs<double> a{av<double>(P, g), g, mag, c->d_part};
+    r = launch(c, "kinetic", k_kinetic_partial<double>, k_kinetic_partial_b<double>, dim3(nb), a);
+  } else {
+    KineticArgs<float> a{av<float>(P, g), g, mag, c->d_part};
+    r = launch(c, "kinetic", k_kinetic_partial<float>, k_kinetic_partial_b<float>, dim3(nb), a);
   }
-  CHECK_LAUNCH(c);
+  if (r) return r;
   return final_reduce(c, c->d_part, nb, 1, 0, slot);
 }
 
-int mom_refresh(mag_ctx c, const Geom& g, int prec, void* P, uint64_t seed, uint64_t epoch, double sig) {
-  KTimer t(c, "mom_refresh");
-  if (prec == MAG_F64)
-    hipLaunchKernelGGL((k_mom_refresh<double>), grid_links(g), dim3(BLOCK), 0, c->stream, av<double>(P, g), g.V, seed,
-                       epoch, sig);
-  else
-    hipLaunchKernelGGL((k_mom_refresh<float>), grid_links(g), dim3(BLOCK), 0, c->stream, av<float>(P, g), g.V, seed,
-                       epoch, sig);
-  CHECK_LAUNCH(c);
-  return MAG_OK;
+int mom_refresh(mag_ctx c, const Geom& g, int prec, void* P, uint64_t seed, uint64_t epoch, double sig,
+                const uint64_t* epoch_ptr) {
+  if (prec == MAG_F64) {
+    MomArgs<double> a{av<double>(P, g), g.V, seed, epoch, epoch_ptr, sig};
+    return launch(c, "mom_refresh", k_mom_refresh<double>, k_mom_refresh_b<double>, grid_links(g), a);
+  }
+  MomArgs<float> a{av<float>(P, g), g.V, seed, epoch, epoch_ptr, sig};
+  return launch(c, "mom_refresh", k_mom_refresh<float>, k_mom_refresh_b<float>, grid_links(g), a);
 }
 
 int mom_update(mag_ctx c, const Geom& g, int prec, void* P, void* F, double eps) {
-  KTimer t(c, "mom_update");
-  if (prec == MAG_F64)
-    hipLaunchKernelGGL((k_mom_update<double>), grid_links(g), dim3(BLOCK), 0, c->stream, av<double>(P, g),
-                       av<double>(F, g), g.V, eps);
-  else
-    hipLaunchKernelGGL((k_mom_update<float>), grid_links(g), dim3(BLOCK), 0, c->stream, av<float>(P, g),
-                       av<float>(F, g), g.V, eps);
-  CHECK_LAUNCH(c);
-  return MAG_OK;
+  if (prec == MAG_F64) {
+    MomUpdateArgs<double> a{av<double>(P, g), av<double>(F, g), g.V, eps};
+    return launch(c, "mom_update", k_mom_update<double>, k_mom_update_b<double>, grid_links(g), a);
+  }
+  MomUpdateArgs<float> a{av<float>(P, g), av<float>(F, g), g.V, eps};
+  return launch(c, "mom_update", k_mom_update<float>, k_mom_update_b<float>, grid_links(g), a);
 }
+
+int copy_dev(mag_ctx c, void* dst, const void* src, size_t bytes) { return copy_async(c, dst, src, bytes); }
+
+int fill_dev(mag_ctx c, double* p, double v, int n) { return fill_async(c, p, v, n); }
 
 int alg_scale(mag_ctx c, const Geom& g, int prec, void* P, double s) {
   if (prec == MAG_F64)

@@ -166,7 +166,7 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 // free links, W = U(x+mu) S(x+mu)^+ - S(x)^+ U(x) for constraint classes 1 and
 // 10, and F = 0 for class 100. Stored as F_a = -(beta/3) Im Tr(T_a W).
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_gauge_force(ForceArgs<TS, NC> a) {
+MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   const int s = xcd_block() * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const int mu = blockIdx.y;
@@ -196,6 +196,10 @@ __global__ __launch_bounds__(BLOCK) void k_gauge_force(ForceArgs<TS, NC> a) {
   for (int k = 0; k < 8; ++k) f[k] *= fac;
   force_epilogue<TS, NC, TC>(a, mu, s, Ux, f);
 }
+template <int ACT, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, TS, NC, TC>(a); }
+template <int ACT, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
 
 // ============================================================ link updates
 // evolve_gauge_field (field-hmc.h:442-482) with the constraint classes:
@@ -212,7 +216,7 @@ struct UpdateArgs {
 };
 
 template <class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) {
+MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const int mu = blockIdx.y;
@@ -237,20 +241,34 @@ __global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) {
   }
   a.Uout.store(mu, s, U);
 }
+template <class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) { k_link_update_body<TS, NC, TC>(a); }
+template <class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_link_update_b(const UpdateArgs<TS, NC>* t) { k_link_update_body<TS, NC, TC>(t[blockIdx.z]); }
 
 // P += eps m F (field-hmc.h:427-440), for the unfused path.
 template <class TS>
-__global__ __launch_bounds__(BLOCK) void k_mom_update(AlgView<TS> P, AlgView<TS> F, int V, double eps) {
+struct MomUpdateArgs {
+  AlgView<TS> P, F;
+  int V;
+  double eps;
+};
+template <class TS>
+MAG_D void k_mom_update_body(const MomUpdateArgs<TS>& a) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
-  if (s >= V) return;
+  if (s >= a.V) return;
   const int mu = blockIdx.y;
   double p[8], f[8];
-  P.load(mu, s, p);
-  F.load(mu, s, f);
+  a.P.load(mu, s, p);
+  a.F.load(mu, s, f);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) p[k] = mfma(eps, f[k], p[k]);
-  P.store(mu, s, p);
+  for (int k = 0; k < 8; ++k) p[k] = mfma(a.eps, f[k], p[k]);
+  a.P.store(mu, s, p);
 }
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_mom_update(MomUpdateArgs<TS> a) { k_mom_update_body<TS>(a); }
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_mom_update_b(const MomUpdateArgs<TS>* t) { k_mom_update_body<TS>(t[blockIdx.z]); }
 
 // ============================================================ stout chain
 // Forward level k (hep-lat/0311018 eqs. 1-3; get_Q, field-multi.h:32-39,
@@ -268,7 +286,7 @@ struct StoutFwdArgs {
 };
 
 template <class TSI, int NCI, class TSO, int NCO, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) {
+MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
   const int s = xcd_block() * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const int mu = blockIdx.y;
@@ -287,6 +305,10 @@ __global__ __launch_bounds__(BLOCK) void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO,
     a.C.store(mu, s, scale(S, rho));
   }
 }
+template <class TSI, int NCI, class TSO, int NCO, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(a); }
+template <class TSI, int NCI, class TSO, int NCO, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_fwd_b(const StoutFwdArgs<TSI, NCI, TSO, NCO>* t) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(t[blockIdx.z]); }
 
 // Lambda step of level k for one link, given Z' = dS/dU_{k+1} there
 // (compute_Lambda, field-multi.h:180-269, eqs. 72-73):
@@ -330,7 +352,7 @@ struct StoutTopArgs {
 };
 
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_top(StoutTopArgs<TS, NC> a) {
+MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
   const int s = xcd_block() * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const int mu = blockIdx.y;
@@ -341,6 +363,10 @@ __global__ __launch_bounds__(BLOCK) void k_stout_top(StoutTopArgs<TS, NC> a) {
   const M3<TC> Z = scale(dag(S), TC(-a.beta / 3.));
   stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Yout, a.Wout, TC(a.rho));
 }
+template <int ACT, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_top(StoutTopArgs<TS, NC> a) { k_stout_top_body<ACT, TS, NC, TC>(a); }
+template <int ACT, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_top_b(const StoutTopArgs<TS, NC>* t) { k_stout_top_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
 
 // Backward level k (eq. 75): Z_k = W_k + sum over nu of the six
 // Lambda-inserted staple terms, with Y = -i rho U^+ Lambda:
@@ -400,7 +426,7 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
 }
 
 template <bool LAST, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_bwd(StoutBwdArgs<TS, NC> a) {
+MAG_D void k_stout_bwd_body(const StoutBwdArgs<TS, NC>& a) {
   const int s = xcd_block() * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const int mu = blockIdx.y;
@@ -417,12 +443,22 @@ __global__ __launch_bounds__(BLOCK) void k_stout_bwd(StoutBwdArgs<TS, NC> a) {
     force_epilogue<TS, NC, TC>(a.f, mu, s, Ux, f);
   }
 }
+template <bool LAST, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_bwd(StoutBwdArgs<TS, NC> a) { k_stout_bwd_body<LAST, TS, NC, TC>(a); }
+template <bool LAST, class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_stout_bwd_b(const StoutBwdArgs<TS, NC>* t) { k_stout_bwd_body<LAST, TS, NC, TC>(t[blockIdx.z]); }
 
 // Constrained runs with smearing: F from a stored Z_0 with the classes of
 // field-hmc.h:353-378: W = U Z (0), U(x+mu) Z(x+mu) - Z U (1, 10), 0 (100);
 // F_a = Im Tr(T_a W).
+template <class TS, int NC>
+struct ForceZArgs {
+  ForceArgs<TS, NC> f;
+  MatView<TS> Z;
+};
 template <class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_force_from_Z(ForceArgs<TS, NC> a, MatView<TS> Z) {
+MAG_D void k_force_from_Z_body(const ForceZArgs<TS, NC>& z) {
+  const ForceArgs<TS, NC>& a = z.f;
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const int mu = blockIdx.y;
@@ -431,33 +467,53 @@ __global__ __launch_bounds__(BLOCK) void k_force_from_Z(ForceArgs<TS, NC> a, Mat
   M3<TC> W = zero3<TC>();
   const M3<TC> Ux = a.U.template load<TC>(mu, s);
   if (cls == 0) {
-    W = mul(Ux, Z.template load<TC>(mu, s));
+    W = mul(Ux, z.Z.template load<TC>(mu, s));
   } else if (cls != 100) {
     const int ys = s + off_rt(a.g, p, mu, 1);
-    W = sub(mul(a.U.template load<TC>(mu, ys), Z.template load<TC>(mu, ys)), mul(Z.template load<TC>(mu, s), Ux));
+    W = sub(mul(a.U.template load<TC>(mu, ys), z.Z.template load<TC>(mu, ys)), mul(z.Z.template load<TC>(mu, s), Ux));
   }
   TC f[8];
   force_alg_from(W, f);
   a.F.store(mu, s, f);
 }
+template <class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_force_from_Z(ForceZArgs<TS, NC> a) { k_force_from_Z_body<TS, NC, TC>(a); }
+template <class TS, int NC, class TC>
+__global__ __launch_bounds__(BLOCK) void k_force_from_Z_b(const ForceZArgs<TS, NC>* t) {
+  k_force_from_Z_body<TS, NC, TC>(t[blockIdx.z]);
+}
 
 // ============================================================ RNG kernels
 // init_momentum (field-hmc.h:646-673): P = sum_a T_a g_a, g_a ~ N(0, sqrt m),
-// drawn from the Philox momentum stream of link (s*4+mu), epoch = trajectory.
+// drawn from the Philox momentum stream of link (s*4+mu), epoch = trajectory
+// (read from device memory when epoch_ptr is set, so a captured trajectory
+// replays with a new epoch).
 template <class TS>
-__global__ __launch_bounds__(BLOCK) void k_mom_refresh(AlgView<TS> P, int V, uint64_t seed, uint64_t epoch,
-                                                       double sig) {
+struct MomArgs {
+  AlgView<TS> P;
+  int V;
+  uint64_t seed, epoch;
+  const uint64_t* epoch_ptr;
+  double sig;
+};
+template <class TS>
+MAG_D void k_mom_refresh_body(const MomArgs<TS>& a) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
-  if (s >= V) return;
+  if (s >= a.V) return;
   const int mu = blockIdx.y;
+  const uint64_t epoch = a.epoch_ptr ? *a.epoch_ptr : a.epoch;
   const uint32_t link = (uint32_t)s * 4u + (uint32_t)mu;
   double g[8];
 #pragma unroll
-  for (uint32_t d = 0; d < 4; ++d) rng_gauss2(rng_draw(seed, RNG_MOMENTUM, epoch, link, d), g[2 * d], g[2 * d + 1]);
+  for (uint32_t d = 0; d < 4; ++d) rng_gauss2(rng_draw(a.seed, RNG_MOMENTUM, epoch, link, d), g[2 * d], g[2 * d + 1]);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) g[k] *= sig;
-  P.store(mu, s, g);
+  for (int k = 0; k < 8; ++k) g[k] *= a.sig;
+  a.P.store(mu, s, g);
 }
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_mom_refresh(MomArgs<TS> a) { k_mom_refresh_body<TS>(a); }
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_mom_refresh_b(const MomArgs<TS>* t) { k_mom_refresh_body<TS>(t[blockIdx.z]); }
 
 // Haar-random SU(3) links (Mezzadri 2007): Gram-Schmidt of a complex Gaussian
 // matrix, det phase divided out.
@@ -544,7 +600,18 @@ MAG_D double block_max(double v) {
 // field-matrix.h:202-221; get_rectangular, field-matrix.h:249-297), one
 // partial pair per block: part[2b] = plaquettes, part[2b+1] = rectangles.
 template <class TS, int NC>
-__global__ __launch_bounds__(BLOCK) void k_action_partial(LinkView<TS, NC> U, Geom g, double* part, int rect) {
+struct ActionArgs {
+  LinkView<TS, NC> U;
+  Geom g;
+  double* part;
+  int rect;
+};
+template <class TS, int NC>
+MAG_D void k_action_partial_body(const ActionArgs<TS, NC>& aa) {
+  const LinkView<TS, NC>& U = aa.U;
+  const Geom& g = aa.g;
+  double* part = aa.part;
+  const int rect = aa.rect;
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   double sp = 0., sr = 0.;
   if (s < g.V) {
@@ -588,10 +655,27 @@ __global__ __launch_bounds__(BLOCK) void k_action_partial(LinkView<TS, NC> U, Ge
     part[2 * blockIdx.x + 1] = tr;
   }
 }
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_action_partial(ActionArgs<TS, NC> a) { k_action_partial_body<TS, NC>(a); }
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_action_partial_b(const ActionArgs<TS, NC>* t) {
+  k_action_partial_body<TS, NC>(t[blockIdx.z]);
+}
 
 // Sum p_a^2 over links of class != 100 (field-hmc.h:596-629), FP64.
 template <class TS>
-__global__ __launch_bounds__(BLOCK) void k_kinetic_partial(AlgView<TS> P, Geom g, int mag, double* part) {
+struct KineticArgs {
+  AlgView<TS> P;
+  Geom g;
+  int mag;
+  double* part;
+};
+template <class TS>
+MAG_D void k_kinetic_partial_body(const KineticArgs<TS>& a) {
+  const AlgView<TS>& P = a.P;
+  const Geom& g = a.g;
+  const int mag = a.mag;
+  double* part = a.part;
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   double acc = 0.;
   if (s < g.V) {
@@ -608,11 +692,26 @@ __global__ __launch_bounds__(BLOCK) void k_kinetic_partial(AlgView<TS> P, Geom g
   const double t = block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_kinetic_partial(KineticArgs<TS> a) { k_kinetic_partial_body<TS>(a); }
+template <class TS>
+__global__ __launch_bounds__(BLOCK) void k_kinetic_partial_b(const KineticArgs<TS>* t) {
+  k_kinetic_partial_body<TS>(t[blockIdx.z]);
+}
 
 // reunitarize (field-matrix.h:83-100): project every link whose first two
 // rows are not all zero, FP64 arithmetic; max Frobenius change per block.
 template <class TS, int NC>
-__global__ __launch_bounds__(BLOCK) void k_reunitarize(LinkView<TS, NC> U, int V, double* part) {
+struct ReunitArgs {
+  LinkView<TS, NC> U;
+  int V;
+  double* part;
+};
+template <class TS, int NC>
+MAG_D void k_reunitarize_body(const ReunitArgs<TS, NC>& a) {
+  const LinkView<TS, NC>& U = a.U;
+  const int V = a.V;
+  double* part = a.part;
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   const int mu = blockIdx.y;
   double dev = 0.;
@@ -636,6 +735,12 @@ __global__ __launch_bounds__(BLOCK) void k_reunitarize(LinkView<TS, NC> U, int V
   }
   const double t = block_max(dev);
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_reunitarize(ReunitArgs<TS, NC> a) { k_reunitarize_body<TS, NC>(a); }
+template <class TS, int NC>
+__global__ __launch_bounds__(BLOCK) void k_reunitarize_b(const ReunitArgs<TS, NC>* t) {
+  k_reunitarize_body<TS, NC>(t[blockIdx.z]);
 }
 
 // Wilson loops (field-wloop.h:26-98): Re Tr of the R x T loop for every
@@ -679,15 +784,57 @@ __global__ __launch_bounds__(BLOCK) void k_wilson_loop(LinkView<TS, NC> U, Wloop
 
 // Final reduction of block partials, one block, fixed order: each thread
 // folds a strided slice, then the block tree. OP 0 = sum, 1 = max.
+struct ReduceArgs {
+  const double* part;
+  int n, stride;
+  double* out;
+};
 template <int OP>
-__global__ __launch_bounds__(BLOCK) void k_final_reduce(const double* part, int n, int stride, double* out) {
+MAG_D void k_final_reduce_body(const ReduceArgs& a) {
   double acc = 0.;
-  for (int i = threadIdx.x; i < n; i += BLOCK) {
-    const double v = part[(size_t)i * stride];
+  for (int i = threadIdx.x; i < a.n; i += BLOCK) {
+    const double v = a.part[(size_t)i * a.stride];
     acc = OP ? fmax(acc, v) : acc + v;
   }
   const double t = OP ? block_max(acc) : block_sum(acc);
-  if (threadIdx.x == 0) *out = t;
+  if (threadIdx.x == 0) *a.out = t;
+}
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_final_reduce(ReduceArgs a) { k_final_reduce_body<OP>(a); }
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_final_reduce_b(const ReduceArgs* t) { k_final_reduce_body<OP>(t[blockIdx.z]); }
+
+// Device copy and fill (used instead of runtime memcpy / memset nodes so a
+// trajectory can be batched over chains and captured in a graph).
+struct CopyArgs {
+  double2* dst;
+  const double2* src;
+  size_t n;  // 16-byte words
+};
+MAG_D void k_copy_body(const CopyArgs& a) {
+  const size_t stride = (size_t)gridDim.x * BLOCK;
+  for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < a.n; i += stride) a.dst[i] = a.src[i];
+}
+template <int DUMMY>
+__global__ __launch_bounds__(BLOCK) void k_copy(CopyArgs a) { k_copy_body(a); }
+template <int DUMMY>
+__global__ __launch_bounds__(BLOCK) void k_copy_b(const CopyArgs* t) { k_copy_body(t[blockIdx.z]); }
+
+struct FillArgs {
+  double* p;
+  double v;
+  int n;
+};
+template <int DUMMY>
+__global__ __launch_bounds__(BLOCK) void k_fill(FillArgs a) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < a.n) a.p[i] = a.v;
+}
+template <int DUMMY>
+__global__ __launch_bounds__(BLOCK) void k_fill_b(const FillArgs* t) {
+  const FillArgs& a = t[blockIdx.z];
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < a.n) a.p[i] = a.v;
 }
 
 // ============================================================ conversions

@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "launch.hpp"
 #include "runtime.hpp"
 
 namespace mag {
@@ -20,12 +21,7 @@ MatView<TS> mview(void* p, const Geom& g) { return MatView<TS>{(TS*)p, g.V}; }
 template <class TS, int NC>
 int Ops<TS, NC>::link_update(mag_ctx c, const Geom& g, void* Uin, void* Uout, void* P, double eps, int mag) {
   UpdateArgs<TS, NC> a{g, lview<TS, NC>(Uin, g), lview<TS, NC>(Uout, g), aview<TS>(P, g), eps, mag};
-  {
-    KTimer t(c, "link_update");
-    hipLaunchKernelGGL((k_link_update<TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, a);
-  }
-  CHECK_LAUNCH(c);
-  return MAG_OK;
+  return launch(c, "link_update", k_link_update<TS, NC, TS>, k_link_update_b<TS, NC, TS>, grid_links(g), a);
 }
 
 // Force of the (possibly smeared) action on U with one of the epilogues
@@ -57,15 +53,11 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   fa.eps = eps;
   const bool iw = act.type == MAG_IWASAKI;
   if (act.n_stout <= 0) {
-    if (iw) {
-      KTimer t(c, "force_iwasaki");
-      hipLaunchKernelGGL((k_gauge_force<ACT_IWASAKI, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fa);
-    } else {
-      KTimer t(c, "force_wilson");
-      hipLaunchKernelGGL((k_gauge_force<ACT_WILSON, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fa);
-    }
-    CHECK_LAUNCH(c);
-    return MAG_OK;
+    if (iw)
+      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, TS, NC, TS>,
+                    k_gauge_force_b<ACT_IWASAKI, TS, NC, TS>, grid_links(g), fa);
+    return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, TS, NC, TS>, k_gauge_force_b<ACT_WILSON, TS, NC, TS>,
+                  grid_links(g), fa);
   }
   const int n = act.n_stout;
   if ((r = ws->need_stout(n))) return r;
@@ -73,11 +65,9 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   for (int k = 0; k < n; ++k) {
     StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g), mview<TS>(ws->Cs[k], g),
                                     lview<TS, NC>(ws->Us[k], g), act.rho, 1};
-    {
-      KTimer t(c, "stout_fwd");
-      hipLaunchKernelGGL((k_stout_fwd<TS, NC, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
-    }
-    CHECK_LAUNCH(c);
+    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_links(g),
+                    fw)))
+      return r;
   }
   int cur = 0;
   {
@@ -91,13 +81,14 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
                             act.beta,
                             act.c1,
                             act.rho};
-    KTimer t(c, "stout_top");
     if (iw)
-      hipLaunchKernelGGL((k_stout_top<ACT_IWASAKI, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, tp);
+      r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS>, k_stout_top_b<ACT_IWASAKI, TS, NC, TS>,
+                 grid_links(g), tp);
     else
-      hipLaunchKernelGGL((k_stout_top<ACT_WILSON, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, tp);
+      r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS>, k_stout_top_b<ACT_WILSON, TS, NC, TS>,
+                 grid_links(g), tp);
+    if (r) return r;
   }
-  CHECK_LAUNCH(c);
   for (int k = n - 1; k >= 1; --k) {
     StoutBwdArgs<TS, NC> bw{};
     bw.g = g;
@@ -110,11 +101,9 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     bw.Yout = mview<TS>(ws->Y[1 - cur], g);
     bw.Wout = mview<TS>(ws->W, g);
     bw.rho = act.rho;
-    {
-      KTimer t(c, "stout_bwd");
-      hipLaunchKernelGGL((k_stout_bwd<false, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, bw);
-    }
-    CHECK_LAUNCH(c);
+    if ((r = launch(c, "stout_bwd", k_stout_bwd<false, TS, NC, TS>, k_stout_bwd_b<false, TS, NC, TS>, grid_links(g),
+                    bw)))
+      return r;
     cur = 1 - cur;
   }
   StoutBwdArgs<TS, NC> bw{};
@@ -128,11 +117,9 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     if ((r = ws->need_Z())) return r;
     bw.Zout = mview<TS>(ws->Z, g);
   }
-  {
-    KTimer t(c, "stout_bwd_force");
-    hipLaunchKernelGGL((k_stout_bwd<true, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, bw);
-  }
-  CHECK_LAUNCH(c);
+  if ((r = launch(c, "stout_bwd_force", k_stout_bwd<true, TS, NC, TS>, k_stout_bwd_b<true, TS, NC, TS>, grid_links(g),
+                  bw)))
+    return r;
   if (!act.mag) return MAG_OK;
   // constrained: F from the stored Z_0 with the link classes
   void* Fout = (out == OUT_F) ? F : nullptr;
@@ -140,13 +127,10 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     if ((r = ws->need_md())) return r;
     Fout = ws->F;
   }
-  ForceArgs<TS, NC> fz = fa;
-  fz.F = aview<TS>(Fout, g);
-  {
-    KTimer t(c, "force_from_z");
-    hipLaunchKernelGGL((k_force_from_Z<TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fz, mview<TS>(ws->Z, g));
-  }
-  CHECK_LAUNCH(c);
+  ForceZArgs<TS, NC> fz{fa, mview<TS>(ws->Z, g)};
+  fz.f.F = aview<TS>(Fout, g);
+  if ((r = launch(c, "force_from_z", k_force_from_Z<TS, NC, TS>, k_force_from_Z_b<TS, NC, TS>, grid_links(g), fz)))
+    return r;
   if (out == OUT_PADD) return mom_update(c, g, std::is_same<TS, double>::value ? MAG_F64 : MAG_F32, P, Fout, eps);
   return MAG_OK;
 }
@@ -195,12 +179,8 @@ int Ops<TS, NC>::action_sums(mag_ctx c, const Geom& g, void* U, bool rect, int s
   const int nb = (g.V + BLOCK - 1) / BLOCK;
   int r;
   if ((r = ensure_part(c, 2 * (size_t)nb))) return r;
-  {
-    KTimer t(c, "action");
-    hipLaunchKernelGGL((k_action_partial<TS, NC>), dim3(nb), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), g,
-                       c->d_part, rect ? 1 : 0);
-  }
-  CHECK_LAUNCH(c);
+  ActionArgs<TS, NC> aa{lview<TS, NC>(U, g), g, c->d_part, rect ? 1 : 0};
+  if ((r = launch(c, "action", k_action_partial<TS, NC>, k_action_partial_b<TS, NC>, dim3(nb), aa))) return r;
   if ((r = final_reduce(c, c->d_part, nb, 2, 0, slot))) return r;
   return final_reduce(c, c->d_part + 1, nb, 2, 0, slot + 1);
 }
@@ -215,28 +195,25 @@ int Ops<TS, NC>::smeared_action_sums(mag_ctx c, const Geom& g, void* U, double r
   if ((r = wsd->need_h(n))) return r;
   for (int k = 0; k < n; ++k) {
     void* dst = wsd->Uh[k & 1];
-    KTimer t(c, "stout_fwd_h");
     if (k == 0) {
       StoutFwdArgs<TS, NC, double, 18> fw{g, lview<TS, NC>(U, g), aview<double>(nullptr, g),
                                           mview<double>(nullptr, g), lview<double, 18>(dst, g), rho, 0};
-      hipLaunchKernelGGL((k_stout_fwd<TS, NC, double, 18, double>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
+      r = launch(c, "stout_fwd_h", k_stout_fwd<TS, NC, double, 18, double>, k_stout_fwd_b<TS, NC, double, 18, double>,
+                 grid_links(g), fw);
     } else {
       StoutFwdArgs<double, 18, double, 18> fw{g, lview<double, 18>(wsd->Uh[(k - 1) & 1], g),
                                               aview<double>(nullptr, g), mview<double>(nullptr, g),
                                               lview<double, 18>(dst, g), rho, 0};
-      hipLaunchKernelGGL((k_stout_fwd<double, 18, double, 18, double>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
+      r = launch(c, "stout_fwd_h", k_stout_fwd<double, 18, double, 18, double>,
+                 k_stout_fwd_b<double, 18, double, 18, double>, grid_links(g), fw);
     }
-    CHECK_LAUNCH(c);
+    if (r) return r;
   }
   void* top = wsd->Uh[(n - 1) & 1];
   const int nb = (g.V + BLOCK - 1) / BLOCK;
   if ((r = ensure_part(c, 2 * (size_t)nb))) return r;
-  {
-    KTimer t(c, "action");
-    hipLaunchKernelGGL((k_action_partial<double, 18>), dim3(nb), dim3(BLOCK), 0, c->stream,
-                       lview<double, 18>(top, g), g, c->d_part, rect ? 1 : 0);
-  }
-  CHECK_LAUNCH(c);
+  ActionArgs<double, 18> aa{lview<double, 18>(top, g), g, c->d_part, rect ? 1 : 0};
+  if ((r = launch(c, "action", k_action_partial<double, 18>, k_action_partial_b<double, 18>, dim3(nb), aa))) return r;
   if ((r = final_reduce(c, c->d_part, nb, 2, 0, slot))) return r;
   return final_reduce(c, c->d_part + 1, nb, 2, 0, slot + 1);
 }
@@ -251,12 +228,12 @@ int Ops<TS, NC>::smear(mag_ctx c, const Geom& g, void* Uout, void* U, double rho
     if (dst == src) dst = (src == Uout) ? ws->Uaux : Uout;
     StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(src, g), aview<TS>(nullptr, g), mview<TS>(nullptr, g),
                                     lview<TS, NC>(dst, g), rho, 0};
-    hipLaunchKernelGGL((k_stout_fwd<TS, NC, TS, NC, TS>), grid_links(g), dim3(BLOCK), 0, c->stream, fw);
-    CHECK_LAUNCH(c);
+    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_links(g),
+                    fw)))
+      return r;
     src = dst;
   }
-  if (src != Uout)
-    HIPCHK(c, hipMemcpyAsync(Uout, src, (size_t)g.V * 4 * NC * sizeof(TS), hipMemcpyDeviceToDevice, c->stream));
+  if (src != Uout) return copy_dev(c, Uout, src, (size_t)g.V * 4 * NC * sizeof(TS));
   return MAG_OK;
 }
 
@@ -265,12 +242,8 @@ int Ops<TS, NC>::reunitarize(mag_ctx c, const Geom& g, void* U, int slot) {
   const int nb = (g.V + BLOCK - 1) / BLOCK;
   int r;
   if ((r = ensure_part(c, 4 * (size_t)nb))) return r;
-  {
-    KTimer t(c, "reunitarize");
-    hipLaunchKernelGGL((k_reunitarize<TS, NC>), grid_links(g), dim3(BLOCK), 0, c->stream, lview<TS, NC>(U, g), g.V,
-                       c->d_part);
-  }
-  CHECK_LAUNCH(c);
+  ReunitArgs<TS, NC> ra{lview<TS, NC>(U, g), g.V, c->d_part};
+  if ((r = launch(c, "reunitarize", k_reunitarize<TS, NC>, k_reunitarize_b<TS, NC>, grid_links(g), ra))) return r;
   return final_reduce(c, c->d_part, 4 * nb, 1, 1, slot);
 }
 

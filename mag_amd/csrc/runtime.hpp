@@ -24,6 +24,7 @@ namespace mag {
 constexpr int RT_BLOCK = 256;
 
 struct Workspace;
+struct Recorder;
 
 struct TimerRec {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -46,6 +47,8 @@ struct mag_ctx_s {
   bool timing = false;
   std::map<std::string, mag::TimerRec> timers;
   std::vector<hipEvent_t> event_pool;
+  mag::Recorder* rec = nullptr;  // set while a serving batch records its launches
+  bool owns_stream = true;
 };
 
 struct mag_field_s {
@@ -281,7 +284,12 @@ int dispatch_link(mag_ctx c, int prec, int kind, F&& f) {
 // (common.hip: format-independent kernels)
 int final_reduce(mag_ctx c, const double* part, int n, int stride, int op, int slot);
 int kinetic_sum(mag_ctx c, const Geom& g, int prec, void* P, int mag, int slot);
-int mom_refresh(mag_ctx c, const Geom& g, int prec, void* P, uint64_t seed, uint64_t epoch, double sig);
+// epoch_ptr (device memory) overrides epoch when set
+int mom_refresh(mag_ctx c, const Geom& g, int prec, void* P, uint64_t seed, uint64_t epoch, double sig,
+                const uint64_t* epoch_ptr = nullptr);
+// kernel copy / fill (recordable; no runtime memcpy or memset nodes)
+int copy_dev(mag_ctx c, void* dst, const void* src, size_t bytes);
+int fill_dev(mag_ctx c, double* p, double v, int n);
 int mom_update(mag_ctx c, const Geom& g, int prec, void* P, void* F, double eps);
 int alg_scale(mag_ctx c, const Geom& g, int prec, void* P, double s);
 int convert_links(mag_ctx c, const Geom& g, int prec_in, int kind_in, void* in, int prec_out, int kind_out, void* out);

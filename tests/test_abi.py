@@ -41,3 +41,14 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libmag_amd.so"))
     with pytest.raises(ImportError):
         _lib.lib()
+
+
+def test_cpp_host_api_compiles():
+    """The C++ mirror of the reference entry points (include/mag_amd.hpp) builds against the C ABI."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    out = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                          os.path.join(ROOT, "tests", "cpp", "chmc_driver.cpp")], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr

@@ -282,3 +282,15 @@ def test_bitwise_determinism(ctx, orc):
         outs.append((ch.get_links(), [x.h_new for x in r]))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert outs[0][1] == outs[1][1]
+
+
+def test_cpp_driver_runs():
+    """The reference-style C++ driver (run_chmc call sequence through include/mag_amd.hpp)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "chmc_driver")
+    assert os.path.exists(exe), "build with make -C mag_amd"
+    out = subprocess.run([exe, "3"], capture_output=True, text=True, timeout=100)
+    assert out.returncode == 0, out.stdout + out.stderr
+    rows = [l for l in out.stdout.splitlines() if l[:1].isdigit()]
+    assert len(rows) == 3 and "Accept Rate" in out.stdout and "demon:" in out.stdout
