@@ -181,6 +181,37 @@ int mag_chain_traj_index(mag_chain c, long* next_traj);
 int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
                              long acc_rej[2]);
 
+/* -------------------------------------------------------------- serving
+ * Continuous batching of independent chains (no counterpart in the
+ * reference, whose statics allow one chain per process, field-hmc.h:911-922).
+ * Chains are admitted in arrival order while their device footprint fits
+ * the budget; every step runs one trajectory of each admitted chain, with
+ * chains of one shape class batched into single launches (optionally
+ * replayed from a HIP graph) and classes overlapping on several streams.
+ * A chain's results are the same alone or in any mix. */
+typedef struct {
+  int dims[4];
+  mag_action act;
+  mag_hmc_params hmc;
+  int ntraj;             /* trajectories to run */
+  uint64_t links_seed;   /* Haar-random start */
+} mag_serve_request;
+
+enum { MAG_CHAIN_PENDING = 0, MAG_CHAIN_RUNNING = 1, MAG_CHAIN_DONE = 2, MAG_CHAIN_FAILED = 3,
+       MAG_CHAIN_CANCELLED = 4 };
+
+int mag_server_footprint(const mag_serve_request* req, size_t* bytes);
+int mag_server_create(mag_ctx ctx, size_t hbm_budget, int max_batch, int use_graphs, mag_server* out);
+int mag_server_destroy(mag_server s);
+int mag_server_submit(mag_server s, const mag_serve_request* req, int* id);
+int mag_server_cancel(mag_server s, int id);
+/* admit, run one trajectory of every running chain, retire finished ones */
+int mag_server_step(mag_server s, int* active, int* pending);
+int mag_server_status(mag_server s, int id, int* state, int* done_traj);
+/* per-trajectory results and completion times (seconds since server creation) */
+int mag_server_results(mag_server s, int id, mag_traj_result* out, double* t_done, int max);
+int mag_server_info(mag_server s, size_t* used_bytes, long* trajectories, double* elapsed_s);
+
 #ifdef __cplusplus
 }
 #endif

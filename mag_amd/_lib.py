@@ -39,11 +39,7 @@ class TrajResult(C.Structure):
 
 class ServeRequest(C.Structure):
     _fields_ = [("dims", C.c_int * 4), ("act", Action), ("hmc", HmcParams), ("ntraj", C.c_int),
-                ("links_seed", C.c_uint64), ("arrival_s", C.c_double)]
-
-
-class ServeTraj(C.Structure):
-    _fields_ = [("result", TrajResult), ("t_done_s", C.c_double)]
+                ("links_seed", C.c_uint64)]
 
 
 _P = C.c_void_p
@@ -96,6 +92,15 @@ SIGNATURES = {
     "mag_chain_demon_state": [_P, _D],
     "mag_chain_traj_index": [_P, _L],
     "mag_demon_microcanonical": [_P, C.c_int, C.c_double, C.c_uint64, _D, _L],
+    "mag_server_footprint": [C.POINTER(ServeRequest), C.POINTER(C.c_size_t)],
+    "mag_server_create": [_P, C.c_size_t, C.c_int, C.c_int, C.POINTER(_P)],
+    "mag_server_destroy": [_P],
+    "mag_server_submit": [_P, C.POINTER(ServeRequest), _I],
+    "mag_server_cancel": [_P, C.c_int],
+    "mag_server_step": [_P, _I, _I],
+    "mag_server_status": [_P, C.c_int, _I, _I],
+    "mag_server_results": [_P, C.c_int, C.POINTER(TrajResult), _D, C.c_int],
+    "mag_server_info": [_P, C.POINTER(C.c_size_t), _L, _D],
 }
 
 

@@ -140,7 +140,7 @@ int mag_ctx_destroy(mag_ctx c) {
   if (c->d_scalar) (void)hipFree(c->d_scalar);
   if (c->h_scalar) (void)hipHostFree(c->h_scalar);
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(c->stream);
+  if (c->owns_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MAG_OK;
 }

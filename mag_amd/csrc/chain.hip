@@ -1,52 +1,37 @@
 // Device-resident HMC chain: run_chmc / update_field (field-hmc.h:770-986).
 // Links, backup, momenta and workspace stay in HBM; each trajectory issues
-// its kernels on the context stream and synchronises once, to read the eight
-// FP64 scalars (H_old parts, H_new parts, plaquettes, reunitarization) that
-// the accept step needs.
+// its kernels on the chain's stream and synchronises once, to read the
+// twelve FP64 scalars (H_old and H_new parts, thin plaquettes,
+// reunitarization deviation) that the accept step needs.
 #include <cmath>
 #include <cstring>
 
-#include "runtime.hpp"
+#include "chain_internal.hpp"
 
 using namespace mag;
 
-struct mag_chain_s {
-  mag_ctx ctx = nullptr;
-  int dims[4] = {0, 0, 0, 0};
-  Geom g;
-  mag_action act{};
-  mag_hmc_params p{};
-  int prec = MAG_F64, kind = MAG_LINK18;
-  mag_field U = nullptr, Ubak = nullptr, P = nullptr;
-  // FP64 fields of the reversibility check (created on first use)
-  mag_field U64 = nullptr, U64b = nullptr, P64 = nullptr;
-  long traj = 0;
-  bool demon_init = false;
-  double demon[4] = {0., 0., 0., 0.};  // S0p, S0r, Ep, Er
-  double emax = 0.;
-  int naccept = 0;
-};
-
-namespace {
+namespace mag {
 
 enum : uint32_t { STREAM_ACCEPT = 3 };
 
-// scalar slots of one trajectory
-enum { S_OLD_P = 0, S_OLD_R = 1, S_OLD_K = 2, S_NEW_P = 4, S_NEW_R = 5, S_NEW_K = 6, S_DEV = 7, S_THIN_OLD = 8,
-       S_THIN_NEW = 10 };
+bool chain_needs_rect(mag_chain c) { return c->act.type == MAG_IWASAKI || c->p.accept_mode != 0; }
+
+namespace {
 
 template <class TS, int NC>
-int traj_kernels(mag_chain c, bool need_rect) {
+int traj_kernels(mag_chain c, bool use_epoch_ptr) {
   using O = Ops<TS, NC>;
   mag_ctx x = c->ctx;
   const Geom& g = c->g;
   Workspace* ws = workspace(x, g, c->prec, NC);
-  const uint64_t i = (uint64_t)c->traj;
+  const bool rect = chain_needs_rect(c);
   int r;
   // init_momentum (field-hmc.h:810)
-  if ((r = mom_refresh(x, g, c->prec, c->P->p, c->p.seed, i, 1.))) return r;
+  if ((r = mom_refresh(x, g, c->prec, c->P->p, c->p.seed, (uint64_t)c->traj, 1.,
+                       use_epoch_ptr ? c->d_epoch : nullptr)))
+    return r;
   // H_old (field-hmc.h:812) and the thin plaquette of the old state
-  if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, need_rect, S_OLD_P))) return r;
+  if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, rect, S_OLD_P))) return r;
   if ((r = kinetic_sum(x, g, c->prec, c->P->p, c->act.mag, S_OLD_K))) return r;
   if ((r = O::action_sums(x, g, c->U->p, false, S_THIN_OLD))) return r;
   // backup, integrate (field-hmc.h:814)
@@ -54,26 +39,25 @@ int traj_kernels(mag_chain c, bool need_rect) {
   if (c->p.integrator == 0) r = O::fg(x, g, c->U->p, c->P->p, c->act, c->p.trajectory_length, c->p.dt, ws);
   else r = O::leapfrog(x, g, c->U->p, c->P->p, c->act, c->p.trajectory_length, c->p.dt, ws);
   if (r) return r;
-  if (c->p.reunitarize) {
-    if ((r = O::reunitarize(x, g, c->U->p, S_DEV))) return r;
-  } else {
-    if ((r = fill_dev(x, x->d_scalar + S_DEV, 0., 1))) return r;
-  }
+  if (c->p.reunitarize) r = O::reunitarize(x, g, c->U->p, S_DEV);
+  else r = fill_dev(x, x->d_scalar + S_DEV, 0., 1);
+  if (r) return r;
   // H_new (field-hmc.h:816)
-  if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, need_rect, S_NEW_P))) return r;
+  if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, rect, S_NEW_P))) return r;
   if ((r = kinetic_sum(x, g, c->prec, c->P->p, c->act.mag, S_NEW_K))) return r;
   return O::action_sums(x, g, c->U->p, false, S_THIN_NEW);
 }
 
-int run_one(mag_chain c, mag_traj_result* out) {
-  mag_ctx x = c->ctx;
-  const bool need_rect = c->act.type == MAG_IWASAKI || c->p.accept_mode != 0;
-  int r = dispatch_link(x, c->prec, c->kind, [&](auto t) -> int {
-    return traj_kernels<typename decltype(t)::T, decltype(t)::N>(c, need_rect);
+}  // namespace
+
+int chain_traj_kernels(mag_chain c, bool use_epoch_ptr) {
+  return dispatch_link(c->ctx, c->prec, c->kind, [&](auto t) -> int {
+    return traj_kernels<typename decltype(t)::T, decltype(t)::N>(c, use_epoch_ptr);
   });
-  if (r) return r;
-  if ((r = fetch_scalars(x, 12))) return r;
-  const double* s = x->h_scalar;
+}
+
+int chain_accept(mag_chain c, const double* s, mag_traj_result* out) {
+  mag_ctx x = c->ctx;
   const double S_old = action_from_sums(c->act, s[S_OLD_P], s[S_OLD_R]);
   const double S_new = action_from_sums(c->act, s[S_NEW_P], s[S_NEW_R]);
   const double h_old = s[S_OLD_K] / 2. + S_old;
@@ -87,11 +71,12 @@ int run_one(mag_chain c, mag_traj_result* out) {
   if (c->p.accept_mode != 0) {
     // demon window on the raw plaquette / rectangle sums (field-demon.h:173-176)
     const double e = c->emax;
-    const bool win = (c->demon[0] - Sp < e) && (c->demon[0] - Sp > -e) && (c->demon[1] - Sr < e) &&
-                     (c->demon[1] - Sr > -e);
+    const bool win =
+        (c->demon[0] - Sp < e) && (c->demon[0] - Sp > -e) && (c->demon[1] - Sr < e) && (c->demon[1] - Sr > -e);
     acc = (c->p.accept_mode == 1) ? win : (acc && win);
   }
-  const bool bad = std::isnan(h_new) || std::isnan(h_old);
+  // a non-finite Hamiltonian is rejected, also during forced accepts
+  const bool bad = !std::isfinite(h_new) || !std::isfinite(h_old);
   if (i < c->p.num_forced_accept && !bad) acc = true;
   if (bad) acc = false;
   if (acc) {
@@ -101,7 +86,8 @@ int run_one(mag_chain c, mag_traj_result* out) {
     }
     c->naccept++;
   } else {
-    HIPCHK(x, hipMemcpyAsync(c->U->p, c->Ubak->p, c->U->bytes, hipMemcpyDeviceToDevice, x->stream));
+    int r = copy_dev(x, c->U->p, c->Ubak->p, c->U->bytes);
+    if (r) return r;
   }
   if (out) {
     out->h_old = h_old;
@@ -120,7 +106,7 @@ int run_one(mag_chain c, mag_traj_result* out) {
   return MAG_OK;
 }
 
-int init_demon(mag_chain c) {
+int chain_init_demon(mag_chain c) {
   if (c->demon_init) return MAG_OK;
   mag_ctx x = c->ctx;
   int r = dispatch_link(x, c->prec, c->kind, [&](auto t) -> int {
@@ -136,7 +122,38 @@ int init_demon(mag_chain c) {
   return MAG_OK;
 }
 
-}  // namespace
+int ctx_child_create(mag_ctx parent, hipStream_t stream, mag_ctx* out) {
+  auto* c = new mag_ctx_s();
+  c->device = parent->device;
+  c->stream = stream;
+  c->owns_stream = false;
+  if (hipMalloc(&c->d_scalar, 16 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->h_scalar, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (c->d_scalar) (void)hipFree(c->d_scalar);
+    delete c;
+    return fail(parent, MAG_ERR_OOM, "child context allocation failed");
+  }
+  c->bytes = 16 * sizeof(double);
+  *out = c;
+  return MAG_OK;
+}
+
+size_t chain_footprint(const int dims[4], const mag_action& act, const mag_hmc_params& p) {
+  const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  const bool f32 = p.precision == MAG_F32;
+  const size_t rs = f32 ? 4 : 8, nc = f32 ? 12 : 18;
+  const size_t link = V * 4 * nc * rs, alg = V * 4 * 8 * rs, mat = V * 4 * 18 * rs, mat64 = V * 4 * 18 * 8;
+  size_t b = 3 * link + 2 * alg;  // U, Ubak, Uaux; P, F
+  if (act.n_stout > 0) {
+    b += (size_t)act.n_stout * (link + alg + mat) + 3 * mat;  // levels; Y[2], W
+    b += (act.n_stout > 1 ? 2 : 1) * mat64;                    // FP64 smearing for H
+    if (act.mag) b += mat;
+  }
+  b += ((V + 255) / 256 * 4 + 4096) * sizeof(double) + 64 * sizeof(double);
+  return b;
+}
+
+}  // namespace mag
 
 extern "C" {
 
@@ -145,7 +162,8 @@ int mag_chain_create(mag_ctx ctx, const int dims[4], const mag_action* act, cons
   if (!ctx || !dims || !act || !p || !out) return MAG_ERR_ARG;
   *out = nullptr;
   if (p->trajectory_length < 0 || !(p->dt >= 0.) || (p->integrator != 0 && p->integrator != 1) ||
-      p->accept_mode < 0 || p->accept_mode > 2 || act->n_stout < 0 || act->mag < 0)
+      p->accept_mode < 0 || p->accept_mode > 2 || act->n_stout < 0 || act->mag < 0 ||
+      (act->type != MAG_WILSON && act->type != MAG_IWASAKI))
     return fail(ctx, MAG_ERR_ARG, "bad chain parameters");
   auto* c = new mag_chain_s();
   c->ctx = ctx;
@@ -158,7 +176,10 @@ int mag_chain_create(mag_ctx ctx, const int dims[4], const mag_action* act, cons
   int r;
   if ((r = mag_field_create(ctx, dims, c->kind, c->prec, &c->U)) ||
       (r = mag_field_create(ctx, dims, c->kind, c->prec, &c->Ubak)) ||
-      (r = mag_field_create(ctx, dims, MAG_ALG8, c->prec, &c->P)) || (r = mag_unit_links(c->U))) {
+      (r = mag_field_create(ctx, dims, MAG_ALG8, c->prec, &c->P)) ||
+      (r = dev_alloc(ctx, (void**)&c->d_epoch, 16)) ||
+      // the largest partial-sum need of a trajectory (reunitarize: 4 per block)
+      (r = ensure_part(ctx, 4 * (size_t)((c->g.V + RT_BLOCK - 1) / RT_BLOCK))) || (r = mag_unit_links(c->U))) {
     mag_chain_destroy(c);
     return r;
   }
@@ -170,6 +191,8 @@ int mag_chain_destroy(mag_chain c) {
   if (!c) return MAG_ERR_ARG;
   for (mag_field f : {c->U, c->Ubak, c->P, c->U64, c->U64b, c->P64})
     if (f) mag_field_destroy(f);
+  if (c->d_epoch) dev_free(c->ctx, c->d_epoch, 16);
+  if (c->owns_ctx) mag_ctx_destroy(c->ctx);
   delete c;
   return MAG_OK;
 }
@@ -200,16 +223,19 @@ int mag_chain_links(mag_chain c, mag_field* U) {
 int mag_chain_run(mag_chain c, int ntraj, mag_traj_result* results) {
   if (!c || ntraj < 0) return MAG_ERR_ARG;
   int r;
-  if (c->p.accept_mode != 0 && (r = init_demon(c))) return r;
-  for (int t = 0; t < ntraj; ++t)
-    if ((r = run_one(c, results ? results + t : nullptr))) return r;
+  if (c->p.accept_mode != 0 && (r = chain_init_demon(c))) return r;
+  for (int t = 0; t < ntraj; ++t) {
+    if ((r = chain_traj_kernels(c, false))) return r;
+    if ((r = fetch_scalars(c->ctx, S_COUNT))) return r;
+    if ((r = chain_accept(c, c->ctx->h_scalar, results ? results + t : nullptr))) return r;
+  }
   return MAG_OK;
 }
 
 int mag_chain_demon_state(mag_chain c, double state[4]) {
   if (!c || !state) return MAG_ERR_ARG;
   int r;
-  if ((r = init_demon(c))) return r;
+  if ((r = chain_init_demon(c))) return r;
   std::memcpy(state, c->demon, sizeof c->demon);
   return MAG_OK;
 }
@@ -240,12 +266,11 @@ int mag_chain_reversibility(mag_chain c, double* max_link_dev, double* dh_roundt
   Workspace* ws = workspace(x, c->g, MAG_F64, 18);
   const int n = c->p.trajectory_length;
   const double dt = c->p.dt;
-  if (c->p.integrator == 0) r = O::fg(x, c->g, c->U64->p, c->P64->p, c->act, n, dt, ws);
-  else r = O::leapfrog(x, c->g, c->U64->p, c->P64->p, c->act, n, dt, ws);
-  if (r || (r = alg_scale(x, c->g, MAG_F64, c->P64->p, -1.))) return r;
-  if (c->p.integrator == 0) r = O::fg(x, c->g, c->U64->p, c->P64->p, c->act, n, dt, ws);
-  else r = O::leapfrog(x, c->g, c->U64->p, c->P64->p, c->act, n, dt, ws);
-  if (r || (r = alg_scale(x, c->g, MAG_F64, c->P64->p, -1.))) return r;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (c->p.integrator == 0) r = O::fg(x, c->g, c->U64->p, c->P64->p, c->act, n, dt, ws);
+    else r = O::leapfrog(x, c->g, c->U64->p, c->P64->p, c->act, n, dt, ws);
+    if (r || (r = alg_scale(x, c->g, MAG_F64, c->P64->p, -1.))) return r;
+  }
   if ((r = mag_hamiltonian(c->U64, c->P64, &c->act, parts, &h1))) return r;
   if ((r = O::max_diff(x, c->g, c->U64->p, c->U64b->p, 0)) || (r = fetch_scalars(x, 1))) return r;
   if (max_link_dev) *max_link_dev = x->h_scalar[0];

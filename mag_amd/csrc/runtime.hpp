@@ -145,6 +145,8 @@ inline void dev_free(mag_ctx c, void* p, size_t bytes) {
 
 inline int ensure_part(mag_ctx c, size_t n) {
   if (n <= c->part_cap) return MAG_OK;
+  // recorded launches keep the buffer's address: it must not move then
+  if (c->rec) return fail(c, MAG_ERR_STATE, "partial-sum buffer too small while recording");
   if (c->d_part) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dev_free(c, c->d_part, c->part_cap * sizeof(double));
