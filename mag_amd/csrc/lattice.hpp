@@ -77,13 +77,23 @@ MAG_HD int link_class_rt(const Site& p, int mu, int mag) {
 }
 
 // ------------------------------------------------------------------ views
+// Field pointers always address device global memory; the explicit address
+// space keeps loads global (not flat) also when a view comes from a table in
+// memory (batched launches).
+template <class T>
+using gp = __attribute__((address_space(1))) T*;
+template <class T>
+MAG_D gp<T> gptr(T* p) { return (gp<T>)p; }
+template <class T>
+MAG_D gp<const T> gptr(const T* p) { return (gp<const T>)p; }
+
 template <class TS, int NC>
 struct LinkView {
   TS* p;
   int V;
   template <class TC>
   MAG_D M3<TC> load(int mu, int s) const {
-    const TS* b = p + (size_t)mu * NC * V + s;
+    const auto b = gptr(p + (size_t)mu * NC * V + s);
     M3<TC> m;
 #pragma unroll
     for (int k = 0; k < NC / 2; ++k) {
@@ -95,7 +105,7 @@ struct LinkView {
   }
   template <class TC>
   MAG_D void store(int mu, int s, const M3<TC>& m) const {
-    TS* b = p + (size_t)mu * NC * V + s;
+    const auto b = gptr(p + (size_t)mu * NC * V + s);
 #pragma unroll
     for (int k = 0; k < NC / 2; ++k) {
       b[(size_t)(2 * k) * V] = TS(m.e[k].r);
@@ -110,13 +120,13 @@ struct AlgView {
   int V;
   template <class TC>
   MAG_D void load(int mu, int s, TC h[8]) const {
-    const TS* b = p + (size_t)mu * 8 * V + s;
+    const auto b = gptr(p + (size_t)mu * 8 * V + s);
 #pragma unroll
     for (int a = 0; a < 8; ++a) h[a] = TC(b[(size_t)a * V]);
   }
   template <class TC>
   MAG_D void store(int mu, int s, const TC h[8]) const {
-    TS* b = p + (size_t)mu * 8 * V + s;
+    const auto b = gptr(p + (size_t)mu * 8 * V + s);
 #pragma unroll
     for (int a = 0; a < 8; ++a) b[(size_t)a * V] = TS(h[a]);
   }
