@@ -52,6 +52,9 @@ def parse():
     p.add_argument("--n-stout", type=int, default=3)
     p.add_argument("--fp64-steps", type=int, default=3, help="trajectories of the FP64 comparison (0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--serve-chains", type=int, default=64,
+                   help="chains of the serving sample (0: skip); the full stream has 512")
+    p.add_argument("--serve-max-traj", type=int, default=10, help="trajectory cap per chain in the sample")
     p.add_argument("--seed", type=int, default=32)
     return p.parse_args()
 
@@ -79,6 +82,21 @@ def cpu_baseline(L_full, n_md, n_stout):
             "sample": f"FP64 oracle on 16^4: 1 force ({t1 - t0:.2f} s), 1 link update ({t2 - t1:.2f} s), "
                       f"1 Hamiltonian ({t3 - t2:.2f} s); scaled x{scale:.0f} by volume to one {L_full}^4 "
                       f"trajectory ({4 * n_md} forces, {4 * n_md + 1} updates, 2 H) = {t_traj:.1f} s"}
+
+
+def serve_sample(M, ctx, nchains, max_traj):
+    """The serving stream (Poisson arrivals at 200/s; volumes {4,6,8,12,16,24}^4; Wilson or Iwasaki;
+    0-3 stout levels; own seeds; FP32 MD, n_md = 10) for its first `nchains` chains with trajectory
+    counts drawn from 5..max_traj, through the continuous-batching server."""
+    from mag_amd import serving as S
+    stream = S.poisson_stream(nchains=nchains, rate=200.0, seed=2026, traj_range=(5, max_traj))
+    srv = S.Server(ctx, hbm_budget=200 << 30, max_batch=64, graphs=True)
+    out, wall = S.drive(srv, stream)
+    st = S.latency_stats(out)
+    srv.close()
+    st.update({"chains": nchains, "wall_s": round(wall, 3), "traj_per_s": round(st["trajectories"] / wall, 2),
+               "sample": f"first {nchains} chains of the 512-chain stream, trajectories 5..{max_traj}"})
+    return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
 
 
 def run_chain(M, ctx, L, arg, seed, precision, steps, warmup, timing):
@@ -151,6 +169,8 @@ def main():
         dt64, r64 = run_chain(M, ctx, L, arg, a.seed, M.F64, a.fp64_steps, 1, False)
         out["fp64_traj_per_s"] = round(a.fp64_steps / dt64, 4)
         out["fp32_over_fp64"] = round(traj_s / (a.fp64_steps / dt64), 3)
+    if a.serve_chains > 0:
+        out["serving"] = serve_sample(M, ctx, a.serve_chains, a.serve_max_traj)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(L, n, a.n_stout)
     print(json.dumps(out))

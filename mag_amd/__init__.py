@@ -7,6 +7,7 @@ are handles; host arrays use the reference's qlat layout, float64 of shape
 (V, 4, 18) = [site][mu][3x3 complex, interleaved re/im].
 """
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field as dfield
 
 import numpy as np
@@ -44,14 +45,22 @@ class Context:
 
     def __init__(self, device=0):
         self._h = C.c_void_p()
+        self._children = weakref.WeakSet()  # fields, chains, servers created on this context
         _check(_lib.lib().mag_ctx_create(device, C.byref(self._h)))
 
     @property
     def handle(self):
         return self._h
 
+    def adopt(self, obj):
+        self._children.add(obj)
+
     def close(self):
+        """Destroy the context after every live object created on it (the C ABI
+        requires fields, chains and servers to go first)."""
         if self._h:
+            for ch in list(self._children):
+                ch._release()
             _lib.lib().mag_ctx_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -94,6 +103,7 @@ class Field:
             self._h, self._owned = C.c_void_p(), True
             _check(_lib.lib().mag_field_create(ctx.handle, (C.c_int * 4)(*self.dims), kind, prec, C.byref(self._h)),
                    ctx.handle)
+            ctx.adopt(self)
 
     @property
     def handle(self):
@@ -103,11 +113,14 @@ class Field:
     def volume(self):
         return int(np.prod(self.dims))
 
+    def _release(self):
+        if self._owned and self._h:
+            _lib.lib().mag_field_destroy(self._h)
+        self._h = C.c_void_p()
+
     def __del__(self):
         try:
-            if self._owned and self._h:
-                _lib.lib().mag_field_destroy(self._h)
-                self._h = C.c_void_p()
+            self._release()
         except Exception:
             pass
 
@@ -259,6 +272,8 @@ def avg_wilson_loop(U, r, T):
 def _batch(ctx, fn, mats, prec):
     arrs = [np.ascontiguousarray(np.asarray(m, dtype=np.complex128)).view(np.float64).reshape(-1, 18) for m in mats]
     n = arrs[0].shape[0]
+    if any(a.shape[0] != n for a in arrs):
+        raise ValueError("batched matrices must have the same count")
     out = np.empty((n, 18))
     _check(fn(ctx.handle, *[_dptr(a) for a in arrs], _dptr(out), n, prec), ctx.handle)
     return out.view(np.complex128).reshape(np.asarray(mats[0]).shape)
@@ -297,12 +312,16 @@ class Chain:
         self._h = C.c_void_p()
         _check(_lib.lib().mag_chain_create(ctx.handle, (C.c_int * 4)(*self.dims), C.byref(self.act),
                                            C.byref(self.params), C.byref(self._h)), ctx.handle)
+        ctx.adopt(self)
+
+    def _release(self):
+        if self._h:
+            _lib.lib().mag_chain_destroy(self._h)
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
-            if self._h:
-                _lib.lib().mag_chain_destroy(self._h)
-                self._h = C.c_void_p()
+            self._release()
         except Exception:
             pass
 

@@ -44,12 +44,16 @@ class Server:
         self._h = C.c_void_p()
         _check(_lib.lib().mag_server_create(ctx.handle, hbm_budget, max_batch, 1 if graphs else 0, C.byref(self._h)),
                ctx.handle)
+        ctx.adopt(self)
         self.ntraj = {}
 
-    def close(self):
+    def _release(self):
         if self._h:
             _lib.lib().mag_server_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
+
+    def close(self):
+        self._release()
 
     def __del__(self):
         try:

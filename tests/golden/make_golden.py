@@ -65,7 +65,7 @@ def accept_sequence():
     L = (4, 4, 4, 4)
     a = O.action(0, 5.7)
     U = O.haar_links(L, 1234)
-    h = O.hmc(trajectory_length=6, dt=1 / 6, num_forced_accept=3, seed=99)
+    h = O.hmc(trajectory_length=3, dt=1 / 3, num_forced_accept=3, seed=99)
     _, res, _ = O.run_hmc(L, U, a, h, 0, 20)
     np.savez_compressed(os.path.join(HERE, "accept_seq_wilson_4x4.npz"),
                         accept=np.array([r.accept for r in res]), h_old=np.array([r.h_old for r in res]),

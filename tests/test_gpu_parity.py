@@ -54,7 +54,7 @@ def test_compute_Lambda_batch(ctx, orc):
     rng = np.random.default_rng(2)
     Q = herm(orc, rng, 0.4, 2048)
     S = rng.normal(size=(2048, 3, 3)) + 1j * rng.normal(size=(2048, 3, 3))
-    U = orc.as_cplx(orc.haar_links(L4, 3)).reshape(-1, 3, 3)[:2048]
+    U = orc.as_cplx(orc.haar_links(L8, 3)).reshape(-1, 3, 3)[:2048]
     ref = orc.compute_Lambda(Q, S, U)
     assert rel(M.compute_Lambda(ctx, Q, S, U, F64), ref) < 1e-12
     assert rel(M.compute_Lambda(ctx, Q, S, U, F32), ref) < 5e-5
@@ -215,7 +215,7 @@ def _chain_vs_oracle(ctx, orc, L, act_kw, h_kw, ntraj, mode=0, emax=0.0, links_s
 
 def test_accept_sequence_20_trajectories(ctx, orc):
     ref, res, Ur, ch = _chain_vs_oracle(ctx, orc, L4, dict(type=0, beta=5.7),
-                                        dict(trajectory_length=6, dt=1 / 6, num_forced_accept=3, seed=99), 20)
+                                        dict(trajectory_length=3, dt=1 / 3, num_forced_accept=3, seed=99), 20)
     assert [r.accept for r in res] == [r.accept for r in ref]
     assert 0 < sum(r.accept for r in ref[3:]) < 17  # both outcomes occur
     for r, q in zip(res, ref):
@@ -294,3 +294,27 @@ def test_cpp_driver_runs():
     assert out.returncode == 0, out.stdout + out.stderr
     rows = [l for l in out.stdout.splitlines() if l[:1].isdigit()]
     assert len(rows) == 3 and "Accept Rate" in out.stdout and "demon:" in out.stdout
+
+
+def test_gpu_against_golden_fixtures(ctx, orc):
+    """The GPU path against the committed fixtures (tests/golden/): C1 trajectory and 20-trajectory sequence."""
+    import os
+    G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    d = np.load(os.path.join(G, "c1_wilson_8x4.npz"))
+    dU = M.Field(ctx, L8, LINK18, F64).haar(1234)
+    assert abs(M.total_plaq(dU) - d["total_plaq"]) < 1e-12 * abs(d["total_plaq"])
+    dP = alg(ctx, L8)
+    M.init_momentum(dP, 1234, 0)
+    arg = M.Arg_chmc(trajectory_length=8, dt=0.125, beta=6.0)
+    h0, _ = M.get_hamiltonian(dU, dP, arg)
+    assert abs(h0 - d["h_old"]) < 1e-12 * abs(d["h_old"])
+    M.force_gradient_integrator(dU, dP, arg)
+    h1, _ = M.get_hamiltonian(dU, dP, arg)
+    assert abs(h1 - d["h_new"]) < 1e-9 * abs(d["h_new"])
+    s = np.load(os.path.join(G, "accept_seq_wilson_4x4.npz"))
+    arg = M.Arg_chmc(trajectory_length=3, dt=1 / 3, beta=5.7, num_forced_accept_step=3)
+    ch = M.Chain(ctx, L4, arg, seed=99)
+    ch.haar(1234)
+    res = ch.run(20)
+    assert [r.accept for r in res] == list(s["accept"])
+    assert np.allclose([r.dh for r in res], s["dh"], rtol=1e-6, atol=1e-9)
