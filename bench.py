@@ -60,28 +60,33 @@ def parse():
 
 
 def cpu_baseline(L_full, n_md, n_stout):
-    """The FP64 CPU oracle on a bounded sample: one force evaluation, one link
-    update and one Hamiltonian of the same action on a 16^4 lattice, scaled by
-    volume to a 32^4 trajectory with the FG schedule (4 n forces, 4 n + 1 link
-    updates, 2 Hamiltonians)."""
+    """The FP64 CPU oracle on a bounded sample of the same workload: on a 16^4
+    lattice with the same action, one warm-up force then the median of 3 force
+    evaluations, 3 link updates and 1 Hamiltonian, scaled by volume to one
+    L^4 trajectory with the FG schedule (4 n forces, 4 n + 1 link updates,
+    2 Hamiltonians)."""
+    import statistics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     Ls = (16, 16, 16, 16)
     a = O.action(1, 2.6, -0.331, n_stout=n_stout, rho=0.1)
     U, P = O.haar_links(Ls, 7), O.init_momentum(Ls, 7, 0)
-    t0 = time.perf_counter()
     O.get_force(Ls, U, a)
-    t1 = time.perf_counter()
-    O.evolve_gauge_field(Ls, U, P, 0.01, 0)
-    t2 = time.perf_counter()
-    O.hamiltonian(Ls, U, P, a)
-    t3 = time.perf_counter()
+
+    def timed(fn):
+        t = time.perf_counter()
+        fn()
+        return time.perf_counter() - t
+
+    tf = statistics.median(timed(lambda: O.get_force(Ls, U, a)) for _ in range(3))
+    tu = statistics.median(timed(lambda: O.evolve_gauge_field(Ls, U, P, 0.01, 0)) for _ in range(3))
+    th = timed(lambda: O.hamiltonian(Ls, U, P, a))
     scale = (L_full ** 4) / (16 ** 4)
-    t_traj = scale * (4 * n_md * (t1 - t0) + (4 * n_md + 1) * (t2 - t1) + 2 * (t3 - t2))
-    return {"value": 1.0 / t_traj, "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"FP64 oracle on 16^4: 1 force ({t1 - t0:.2f} s), 1 link update ({t2 - t1:.2f} s), "
-                      f"1 Hamiltonian ({t3 - t2:.2f} s); scaled x{scale:.0f} by volume to one {L_full}^4 "
-                      f"trajectory ({4 * n_md} forces, {4 * n_md + 1} updates, 2 H) = {t_traj:.1f} s"}
+    t_traj = scale * (4 * n_md * tf + (4 * n_md + 1) * tu + 2 * th)
+    return {"value": round(1.0 / t_traj, 8), "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"FP64 oracle on 16^4, same action: force {tf:.2f} s, link update {tu:.3f} s, "
+                      f"Hamiltonian {th:.2f} s (medians of 3); scaled x{scale:.0f} by volume to one {L_full}^4 "
+                      f"trajectory ({4 * n_md} forces, {4 * n_md + 1} updates, 2 H) = {t_traj:.0f} s"}
 
 
 def serve_sample(M, ctx, nchains, max_traj):
