@@ -15,6 +15,7 @@ measured with HIP events on the library's stream, and the CPU oracle timed on
 a bounded sample of the same workload on the host cores.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -28,25 +29,37 @@ FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 M_FLOPS = 198  # one 3x3 complex matrix product
 
-# Algorithmic per-link figures of each kernel family, FP32 storage with
-# 12-real links (bytes: every operand read once and written once; flops:
-# canonical 3x3 products x 198).
+# Per-link figures of each kernel family, FP32 storage with 12-real links.
+# Bytes: every operand the kernel touches read once and written once. Flops:
+# the canonical per-link counts of the force in units of one 3x3 complex
+# product (M = 198 flops): Iwasaki staples + force 85M, each stout level
+# 15M forward and 60M backward (310M for 3 levels). The backward chain is
+# pipelined across kernels: "stout_top" holds the Iwasaki force of the
+# smeared links plus the Lambda step of the top level, each "stout_bwd" the
+# six-term sum of level k plus the Lambda step of level k-1 (one full
+# backward level), "stout_bwd_force" the six-term sum of level 0 and the
+# momentum update. The Lambda step is 1/6 of a backward level (10M), the
+# six-term sum 5/6 (50M), in proportion to the products each executes.
 KERNEL_UNITS = {
     # family: (bytes per link, 3x3 products per link)
-    "stout_fwd": (48 + 32 + 72 + 48, 15),          # read U_k; write Q_k, C_k, U_k+1
-    "stout_top": (48 + 48 + 32 + 72 + 72 + 72, 85 + 6),   # Iwasaki staples of U_n + Lambda of level n-1
-    "stout_bwd": (48 + 72 + 72 + 48 + 32 + 72 + 72 + 72, 54 + 6),  # six-term chain + next Lambda
-    "stout_bwd_force": (48 + 72 + 72 + 32 + 32, 54 + 1),  # six-term chain at level 0 + P += eps F
+    "stout_fwd": (48 + 48 + 32 + 72, 15),          # read U_k; write U_k+1, Q_k, C_k
+    "stout_top": (48 + 48 + 32 + 72 + 32 + 72, 85 + 10),   # read U_n, U_n-1, Q, C; write Lambda, W
+    "stout_bwd": (48 + 32 + 72 + 48 + 32 + 72 + 32 + 72, 60),  # read U_k, Lambda_k, W_k, U, Q, C_k-1; write Lambda, W
+    "stout_bwd_force": (48 + 32 + 72 + 32 + 32, 50),  # read U_0, Lambda_0, W_0, P; write P
     "link_update": (48 + 32 + 48, 3),
     "force_iwasaki": (48 + 32 + 32, 85),
 }
+# Canonical bytes per link of one Iwasaki + 3-stout force evaluation with
+# FP32 12-real thin links (the numerator of "force GB/s").
+FORCE_BYTES_LINK_C3 = 1520
+FORCE_PRODUCTS_LINK = {0: 85, 1: 160, 2: 235, 3: 310}
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=10, help="forced-accept thermalisation trajectories (untimed)")
     p.add_argument("--L", type=int, default=32)
     p.add_argument("--n-md", type=int, default=12)
     p.add_argument("--n-stout", type=int, default=3)
@@ -56,6 +69,8 @@ def parse():
                    help="chains of the serving sample (0: skip); the full stream has 512")
     p.add_argument("--serve-max-traj", type=int, default=10, help="trajectory cap per chain in the sample")
     p.add_argument("--seed", type=int, default=32)
+    p.add_argument("--no-reversibility", dest="reversibility", action="store_false",
+                   help="skip the FP64 reversibility round trip after the timed run")
     return p.parse_args()
 
 
@@ -104,10 +119,38 @@ def serve_sample(M, ctx, nchains, max_traj):
     return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
 
 
-def run_chain(M, ctx, L, arg, seed, precision, steps, warmup, timing):
-    ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005)
+def traffic_bytes(kernel, L):
+    """HBM bytes per launch of `kernel` at L^4 from the calibrated rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary kept in profiles/traffic.json (None if
+    that kernel and lattice were not profiled)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+        e = t["kernels"][kernel]
+        if e["lattice"] != L:
+            return None
+        return int(e["fetch_bytes"] + e["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def thermalize(M, ctx, L, arg, seed, ntraj):
+    """Untimed warm-up: Haar start, `ntraj` FP32 trajectories of the same
+    workload with forced accepts. Returns the chain holding the links."""
+    ch = M.Chain(ctx, (L,) * 4, dataclasses.replace(arg, num_forced_accept_step=ntraj), seed=seed, precision=M.F32)
     ch.haar(seed)
-    ch.run(warmup)
+    ch.run(ntraj)
+    ctx.synchronize()
+    return ch
+
+
+def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing):
+    """A fresh chain (trajectory 0, Metropolis + demon window of 0.5% of the
+    initial plaquette sum) from the links of `start`: `steps` trajectories,
+    each timed on its own. Returns (seconds per trajectory, results, chain)."""
+    ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005)
+    ch.links().copy_from(start.links())
+    ch.demon_state()  # window centre from the starting links, outside the timed region
     ctx.synchronize()
     if timing:
         ctx.reset_timing()
@@ -117,7 +160,7 @@ def run_chain(M, ctx, L, arg, seed, precision, steps, warmup, timing):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
-    return dt, res
+    return dt, res, ch
 
 
 def main():
@@ -129,12 +172,14 @@ def main():
     ctx = M.Context(0)
     L, n = a.L, a.n_md
     arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=2.6, gauge=M.Gauge(M.IWASAKI, -0.331), n_stout=a.n_stout,
-                     rho=0.1, num_forced_accept_step=a.warmup)
+                     rho=0.1, num_forced_accept_step=0)
     V = L ** 4
     nlinks = 4 * V
     print(f"bench: {L}^4 Iwasaki+{a.n_stout}stout FP32-MD, warmup {a.warmup}, steps {a.steps}", file=sys.stderr,
           flush=True)
-    dt, res = run_chain(M, ctx, L, arg, a.seed, M.F32, a.steps, a.warmup, True)
+    therm = thermalize(M, ctx, L, arg, a.seed, a.warmup)
+    dt, res, ch32 = timed_chain(M, ctx, L, arg, a.seed, M.F32, therm, a.steps, True)
+    print(f"bench: FP32 {a.steps} trajectories in {dt:.3f} s", file=sys.stderr, flush=True)
     fams = {}
     for f in list(KERNEL_UNITS) + ["stout_fwd_h", "action", "kinetic", "reduce", "mom_refresh", "reunitarize"]:
         ms, nl = ctx.kernel_time(f)
@@ -144,10 +189,16 @@ def main():
     nforce = 4 * n * a.steps
     force_ms = sum(fams.get(f, (0, 0))[0] for f in ("stout_fwd", "stout_top", "stout_bwd", "stout_bwd_force",
                                                      "force_iwasaki"))
-    force_bytes_link = sum(KERNEL_UNITS[f][0] * (a.n_stout if f == "stout_fwd" else
-                                                 (a.n_stout - 1 if f == "stout_bwd" else 1))
-                           for f in ("stout_fwd", "stout_top", "stout_bwd", "stout_bwd_force"))
-    force_gbs = force_bytes_link * nlinks * nforce / (force_ms * 1e-3) / 1e9 if force_ms else None
+    force_s = force_ms * 1e-3
+    if a.n_stout == 3:
+        force_bytes_link = FORCE_BYTES_LINK_C3
+    else:
+        force_bytes_link = sum(KERNEL_UNITS[f][0] * (a.n_stout if f == "stout_fwd" else
+                                                     (a.n_stout - 1 if f == "stout_bwd" else 1))
+                               for f in ("stout_fwd", "stout_top", "stout_bwd", "stout_bwd_force"))
+    force_gbs = force_bytes_link * nlinks * nforce / force_s / 1e9 if force_ms else None
+    force_tflops = (FORCE_PRODUCTS_LINK.get(a.n_stout, 85 + 75 * a.n_stout) * M_FLOPS * nlinks * nforce / force_s
+                    / 1e12 if force_ms else None)
     dom = max((f for f in fams if f in KERNEL_UNITS), key=lambda f: fams[f][0])
     ms, nl = fams[dom]
     avg_s = ms * 1e-3 / nl
@@ -155,25 +206,41 @@ def main():
     gbs = b_link * nlinks / avg_s / 1e9
     tflops = m_link * M_FLOPS * nlinks / avg_s / 1e12
     roof = {"bound": "valu", "kernel": dom, "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic_bytes(dom, L),
             "hbm_achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
             "avg_launch_ms": round(avg_s * 1e3, 4)}
     out = {"metric": "traj/s + force GB/s at 32^4 Iwasaki+3stout, FP32-MD vs FP64; chain p99 latency",
            "value": round(traj_s, 4), "unit": "traj/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (Haar-random links, seed %d)" % a.seed,
-           "config": {"workload": f"C3: {L}^4 Iwasaki(c1=-0.331, beta=2.6) + {a.n_stout}-stout(rho=0.1), "
+           "config": {"workload": f"{L}^4 Iwasaki(c1=-0.331, beta=2.6) + {a.n_stout}-stout(rho=0.1), "
                                   f"FG tau=1 n={n}, FP32 12-real links, FP64 dH, demon window 0.5%",
                       "lattice": [L] * 4, "parallelism": "single GPU"},
            "link_updates_per_s": round(4 * V * n * traj_s, 1),
            "force_gbs": round(force_gbs, 1) if force_gbs else None,
+           "force_tflops": round(force_tflops, 2) if force_tflops else None,
            "accept_rate": round(sum(r.accept for r in res) / len(res), 3),
+           "dh": [round(r.dh, 5) for r in res],
            "kernel_ms_per_traj": {f: round(v[0] / a.steps, 3) for f, v in fams.items()},
            "roofline": roof}
     if a.fp64_steps > 0:
-        dt64, r64 = run_chain(M, ctx, L, arg, a.seed, M.F64, a.fp64_steps, 1, False)
+        dt64, r64, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.fp64_steps, False)
         out["fp64_traj_per_s"] = round(a.fp64_steps / dt64, 4)
         out["fp32_over_fp64"] = round(traj_s / (a.fp64_steps / dt64), 3)
+        out["fp64_accept_rate"] = round(sum(r.accept for r in r64) / len(r64), 3)
+        m = min(len(r64), len(res))
+        same = 0
+        while same < m and res[same].accept == r64[same].accept:
+            same += 1
+        # While the accept decisions agree the two chains follow the same
+        # trajectories (same momenta), so their dH are comparable one by one.
+        out["dh_fp32_vs_fp64"] = {"trajectories": same,
+                                  "max_abs": max((abs(res[i].dh - r64[i].dh) for i in range(same)), default=None),
+                                  "fp64_dh": [round(r.dh, 5) for r in r64]}
+        print(f"bench: FP64 {a.fp64_steps} trajectories in {dt64:.3f} s", file=sys.stderr, flush=True)
+    if a.reversibility:
+        dev, dh_rt = ch32.reversibility()
+        out["reversibility_fp64"] = {"max_link_dev": dev, "dh_roundtrip": dh_rt}
     if a.serve_chains > 0:
         out["serving"] = serve_sample(M, ctx, a.serve_chains, a.serve_max_traj)
     if not a.no_cpu_baseline:

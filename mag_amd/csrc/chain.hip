@@ -145,7 +145,7 @@ size_t chain_footprint(const int dims[4], const mag_action& act, const mag_hmc_p
   const size_t link = V * 4 * nc * rs, alg = V * 4 * 8 * rs, mat = V * 4 * 18 * rs, mat64 = V * 4 * 18 * 8;
   size_t b = 3 * link + 2 * alg;  // U, Ubak, Uaux; P, F
   if (act.n_stout > 0) {
-    b += (size_t)act.n_stout * (link + alg + mat) + 3 * mat;  // levels; Y[2], W
+    b += (size_t)act.n_stout * (link + alg + mat) + 2 * alg + mat;  // levels; Lambda[2], W
     b += (act.n_stout > 1 ? 2 : 1) * mat64;                    // FP64 smearing for H
     if (act.mag) b += mat;
   }

@@ -312,17 +312,18 @@ __global__ __launch_bounds__(BLOCK) void k_stout_fwd_b(const StoutFwdArgs<TSI, N
 
 // Lambda step of level k for one link, given Z' = dS/dU_{k+1} there
 // (compute_Lambda, field-multi.h:180-269, eqs. 72-73):
-//   Lambda = HT(Gamma), Y_k = -i rho U_k^+ Lambda,
-//   W_k = Z' e^{iQ_k} + i C_k^+ Lambda  (the local part of Z_k, eq. 75).
+//   Lambda_k = HT(Gamma)   (stored as its 8 generator components),
+//   W_k = Z' e^{iQ_k} + i C_k^+ Lambda_k   (the local part of Z_k, eq. 75).
 template <class TS, int NC, class TC>
 MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS, NC>& U, const AlgView<TS>& Qv,
-                             const MatView<TS>& Cv, const MatView<TS>& Yout, const MatView<TS>& Wout, TC rho) {
+                             const MatView<TS>& Cv, const AlgView<TS>& Lout, const MatView<TS>& Wout) {
   const M3<TC> Uk = U.template load<TC>(mu, s);
   const M3<TC> Q = Qv.template load_mat<TC>(mu, s);
   const M3<TC> X = mul(Uk, Zp);
   TC lam[8];
   M3<TC> eQ;
   ch_lambda(Q, X, lam, eQ);
+  Lout.store(mu, s, lam);
   const M3<TC> Lm = alg_to_mat(lam);
   M3<TC> W = mul(Zp, eQ);
   // + i C^+ Lambda
@@ -330,12 +331,6 @@ MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS,
 #pragma unroll
   for (int k = 0; k < 9; ++k) W.e[k] = W.e[k] + cpx<TC>{-CL.e[k].i, CL.e[k].r};
   Wout.store(mu, s, W);
-  // Y = -i rho U^+ Lambda
-  const M3<TC> UL = mul_dn(Uk, Lm);
-  M3<TC> Y;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) Y.e[k] = cpx<TC>{rho * UL.e[k].i, -rho * UL.e[k].r};
-  Yout.store(mu, s, Y);
 }
 
 // Top of the chain: Z_n = -(beta/3) S(U_n)^+ (derivative of the gauge action
@@ -347,7 +342,8 @@ struct StoutTopArgs {
   LinkView<TS, NC> Ulo;     // U_{n-1}
   AlgView<TS> Qlo;
   MatView<TS> Clo;
-  MatView<TS> Yout, Wout;
+  AlgView<TS> Lout;
+  MatView<TS> Wout;
   double beta, c1, rho;
 };
 
@@ -361,7 +357,7 @@ MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
   const Site p = site_of(a.g, s);
   const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.Un, a.g, p, mu, c0, c1);
   const M3<TC> Z = scale(dag(S), TC(-a.beta / 3.));
-  stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Yout, a.Wout, TC(a.rho));
+  stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
 }
 template <int ACT, class TS, int NC, class TC>
 __global__ __launch_bounds__(BLOCK) void k_stout_top(StoutTopArgs<TS, NC> a) { k_stout_top_body<ACT, TS, NC, TC>(a); }
@@ -369,59 +365,98 @@ template <int ACT, class TS, int NC, class TC>
 __global__ __launch_bounds__(BLOCK) void k_stout_top_b(const StoutTopArgs<TS, NC>* t) { k_stout_top_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
 
 // Backward level k (eq. 75): Z_k = W_k + sum over nu of the six
-// Lambda-inserted staple terms, with Y = -i rho U^+ Lambda:
-//   U_nu(x+mu) [U_mu(x+nu)^+ Y_nu(x) + Y_mu(x+nu) U_nu(x)^+]
-// + U_nu(x-nu+mu)^+ [U_mu(x-nu)^+ Y_nu(x-nu)^+ + Y_mu(x-nu) U_nu(x-nu)]
-// + Y_nu(x+mu)^+ (U_nu(x) U_mu(x+nu))^+ + Y_nu(x+mu-nu) U_mu(x-nu)^+ U_nu(x-nu)
-// Then either the Lambda step of level k-1 (LAST = false) or, at k = 0, the
-// thin-link force F_a = Im Tr(T_a U Z_0) with a force epilogue.
+// Lambda-inserted staple terms. With Y = -i rho U^+ Lambda every Y sits next
+// to the link it was built from, so the terms are written in Lambda (8 reals
+// per link instead of 18) around the shared products
+// D = U_nu(x+mu) U_mu(x+nu)^+ and B = U_mu(x-nu) U_nu(x-nu+mu):
+//   upper: i rho [(Lam_nu(x+mu) D - D Lam_mu(x+nu)) U_nu(x)^+ - D U_nu(x)^+ Lam_nu(x)]
+//   lower: i rho U_nu(x+mu-nu)^+ [U_mu(x-nu)^+ (Lam_nu - Lam_mu)(x-nu)
+//                                 - Lam_nu(x+mu-nu) U_mu(x-nu)^+] U_nu(x-nu)
+// 10 products per plane. Then either the Lambda step of level k-1
+// (LAST = false) or, at k = 0, the thin-link force F_a = Im Tr(T_a U Z_0)
+// with a force epilogue.
 template <class TS, int NC>
 struct StoutBwdArgs {
   Geom g;
   LinkView<TS, NC> U;   // U_k
-  MatView<TS> Y, W;     // level k
+  AlgView<TS> L;        // Lambda_k
+  MatView<TS> W;        // local part of Z_k
   // k > 0: Lambda step of level k-1
   LinkView<TS, NC> Ulo;
   AlgView<TS> Qlo;
   MatView<TS> Clo;
-  MatView<TS> Yout, Wout;
+  AlgView<TS> Lout;
+  MatView<TS> Wout;
   double rho;
   // k == 0: force output
   ForceArgs<TS, NC> f;
   MatView<TS> Zout;  // constrained runs: Z_0 is stored for k_force_from_Z
 };
 
+// c -= a b
+template <class T>
+MAG_HD void mul_sub(M3<T>& c, const M3<T>& a, const M3<T>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = c(i, j);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s = cfma(cpx<T>{-a(i, k).r, -a(i, k).i}, b(k, j), s);
+      c(i, j) = s;
+    }
+}
+// c -= a b^+
+template <class T>
+MAG_HD void mul_nd_sub(M3<T>& c, const M3<T>& a, const M3<T>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = c(i, j);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s = cfma(cpx<T>{-a(i, k).r, -a(i, k).i}, cconj(b(j, k)), s);
+      c(i, j) = s;
+    }
+}
+
 template <class TS, int NC, class TC>
 MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int mu) {
   const Geom& g = a.g;
-  M3<TC> Z = a.W.template load<TC>(mu, p.s);
+  M3<TC> acc = zero3<TC>();
   const int xpm = p.s + off_rt(g, p, mu, 1);
 #pragma unroll 1
   for (int i = 0; i < 3; ++i) {
     const int nu = i + (i >= mu ? 1 : 0);
     const int opn = off_rt(g, p, nu, 1), omn = off_rt(g, p, nu, -1);
-    const int xpn = p.s + opn, xmn = p.s + omn;
+    const int xpn = p.s + opn, xmn = p.s + omn, xpmmn = xpm + omn;
     {
-      const M3<TC> Um_xpn = a.U.template load<TC>(mu, xpn);
+      const M3<TC> D = mul_nd(a.U.template load<TC>(nu, xpm), a.U.template load<TC>(mu, xpn));
+      M3<TC> G = mul(a.L.template load_mat<TC>(nu, xpm), D);
+      mul_sub(G, D, a.L.template load_mat<TC>(mu, xpn));
       const M3<TC> Un_x = a.U.template load<TC>(nu, p.s);
-      M3<TC> T = mul_dn(Um_xpn, a.Y.template load<TC>(nu, p.s));
-      mul_nd_acc(T, a.Y.template load<TC>(mu, xpn), Un_x);
-      mul_acc(Z, a.U.template load<TC>(nu, xpm), T);
-      // Y_nu(x+mu)^+ (U_nu(x) U_mu(x+nu))^+
-      const M3<TC> A = mul(Un_x, Um_xpn);
-      Z = add(Z, mul_dd(a.Y.template load<TC>(nu, xpm), A));
+      mul_nd_acc(acc, G, Un_x);
+      const M3<TC> E = mul_nd(D, Un_x);
+      mul_sub(acc, E, a.L.template load_mat<TC>(nu, p.s));
     }
     {
       const M3<TC> Um_xmn = a.U.template load<TC>(mu, xmn);
-      const M3<TC> Un_xmn = a.U.template load<TC>(nu, xmn);
-      M3<TC> T = mul_nd(dag(Um_xmn), a.Y.template load<TC>(nu, xmn));
-      mul_acc(T, a.Y.template load<TC>(mu, xmn), Un_xmn);
-      mul_dn_acc(Z, a.U.template load<TC>(nu, xpm + omn), T);
-      // Y_nu(x+mu-nu) U_mu(x-nu)^+ U_nu(x-nu)
-      const M3<TC> B = mul_dn(Um_xmn, Un_xmn);
-      mul_acc(Z, a.Y.template load<TC>(nu, xpm + omn), B);
+      TC ln[8], lm[8];
+      a.L.load(nu, xmn, ln);
+      a.L.load(mu, xmn, lm);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) ln[k] -= lm[k];
+      M3<TC> K = mul_dn(Um_xmn, alg_to_mat(ln));
+      mul_nd_sub(K, a.L.template load_mat<TC>(nu, xpmmn), Um_xmn);
+      const M3<TC> K2 = mul_dn(a.U.template load<TC>(nu, xpmmn), K);
+      mul_acc(acc, K2, a.U.template load<TC>(nu, xmn));
     }
   }
+  // Z = W + i rho acc
+  M3<TC> Z = a.W.template load<TC>(mu, p.s);
+  const TC rho = TC(a.rho);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Z.e[k] = {mfma(-rho, acc.e[k].i, Z.e[k].r), mfma(rho, acc.e[k].r, Z.e[k].i)};
   return Z;
 }
 
@@ -433,7 +468,7 @@ MAG_D void k_stout_bwd_body(const StoutBwdArgs<TS, NC>& a) {
   const Site p = site_of(a.g, s);
   const M3<TC> Z = stout_six_terms<TS, NC, TC>(a, p, mu);
   if (!LAST) {
-    stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Yout, a.Wout, TC(a.rho));
+    stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
   } else if (a.f.mag) {
     a.Zout.store(mu, s, Z);
   } else {

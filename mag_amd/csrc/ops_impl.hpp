@@ -76,7 +76,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
                             lview<TS, NC>(Ulev(n - 1), g),
                             aview<TS>(ws->Qs[n - 1], g),
                             mview<TS>(ws->Cs[n - 1], g),
-                            mview<TS>(ws->Y[cur], g),
+                            aview<TS>(ws->L[cur], g),
                             mview<TS>(ws->W, g),
                             act.beta,
                             act.c1,
@@ -93,12 +93,12 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     StoutBwdArgs<TS, NC> bw{};
     bw.g = g;
     bw.U = lview<TS, NC>(Ulev(k), g);
-    bw.Y = mview<TS>(ws->Y[cur], g);
+    bw.L = aview<TS>(ws->L[cur], g);
     bw.W = mview<TS>(ws->W, g);
     bw.Ulo = lview<TS, NC>(Ulev(k - 1), g);
     bw.Qlo = aview<TS>(ws->Qs[k - 1], g);
     bw.Clo = mview<TS>(ws->Cs[k - 1], g);
-    bw.Yout = mview<TS>(ws->Y[1 - cur], g);
+    bw.Lout = aview<TS>(ws->L[1 - cur], g);
     bw.Wout = mview<TS>(ws->W, g);
     bw.rho = act.rho;
     if ((r = launch(c, "stout_bwd", k_stout_bwd<false, TS, NC, TS>, k_stout_bwd_b<false, TS, NC, TS>, grid_links(g),
@@ -109,7 +109,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   StoutBwdArgs<TS, NC> bw{};
   bw.g = g;
   bw.U = lview<TS, NC>(U, g);
-  bw.Y = mview<TS>(ws->Y[cur], g);
+  bw.L = aview<TS>(ws->L[cur], g);
   bw.W = mview<TS>(ws->W, g);
   bw.rho = act.rho;
   bw.f = fa;

@@ -181,7 +181,7 @@ struct Workspace {
   void* F = nullptr;     // alg8: force (unfused paths)
   void* Z = nullptr;     // mat18: Z_0 of constrained smeared runs
   std::vector<void*> Us, Qs, Cs;  // stout levels: U_{k+1}, Q_k, C_k
-  void* Y[2] = {nullptr, nullptr};
+  void* L[2] = {nullptr, nullptr};  // stout Lambda (alg8), ping-pong between levels
   void* W = nullptr;
   void* Uh[2] = {nullptr, nullptr};  // FP64 smeared fields for the Hamiltonian
 
@@ -215,8 +215,8 @@ struct Workspace {
       if ((r = get(&Qs[k], alg_bytes()))) return r;
       if ((r = get(&Cs[k], mat_bytes()))) return r;
     }
-    if ((r = get(&Y[0], mat_bytes()))) return r;
-    if ((r = get(&Y[1], mat_bytes()))) return r;
+    if ((r = get(&L[0], alg_bytes()))) return r;
+    if ((r = get(&L[1], alg_bytes()))) return r;
     return get(&W, mat_bytes());
   }
   int need_Z() { return get(&Z, mat_bytes()); }
