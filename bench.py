@@ -59,11 +59,12 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=10, help="forced-accept thermalisation trajectories (untimed)")
+    p.add_argument("--warmup", type=int, default=30, help="forced-accept thermalisation trajectories (untimed)")
     p.add_argument("--L", type=int, default=32)
     p.add_argument("--n-md", type=int, default=12)
     p.add_argument("--n-stout", type=int, default=3)
-    p.add_argument("--fp64-steps", type=int, default=3, help="trajectories of the FP64 comparison (0: skip)")
+    p.add_argument("--fp64-steps", type=int, default=-1,
+                   help="trajectories of the FP64 comparison chain (default: as many as --steps; 0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--serve-chains", type=int, default=64,
                    help="chains of the serving sample (0: skip); the full stream has 512")
@@ -165,6 +166,8 @@ def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing):
 
 def main():
     a = parse()
+    if a.fp64_steps < 0:
+        a.fp64_steps = a.steps
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)", file=sys.stderr)
         sys.exit(2)
@@ -234,8 +237,9 @@ def main():
             same += 1
         # While the accept decisions agree the two chains follow the same
         # trajectories (same momenta), so their dH are comparable one by one.
-        out["dh_fp32_vs_fp64"] = {"trajectories": same,
-                                  "max_abs": max((abs(res[i].dh - r64[i].dh) for i in range(same)), default=None),
+        mx = max((abs(res[i].dh - r64[i].dh) for i in range(same)), default=None)
+        out["dh_fp32_vs_fp64"] = {"trajectories": same, "max_abs": mx,
+                                  "max_rel_to_h": mx / abs(r64[0].h_old) if mx is not None else None,
                                   "fp64_dh": [round(r.dh, 5) for r in r64]}
         print(f"bench: FP64 {a.fp64_steps} trajectories in {dt64:.3f} s", file=sys.stderr, flush=True)
     if a.reversibility:

@@ -4,25 +4,39 @@
 // (field-hmc.h:82-123), DexpiQ (field-multi.h:90-178) and compute_Lambda
 // (field-multi.h:180-269).
 //
-// exp(iQ) = f0 + f1 Q + f2 Q^2. The scalar coefficients are always evaluated
-// in FP64 from the invariants c0 = Re Tr Q^3 / 3 and c1 = Re Tr Q^2 / 2, also
-// on the FP32 molecular-dynamics path: the h_j and b_ij formulas cancel O(1)
-// terms down to O(u^2) (and b_ij divide by (9u^2 - w^2)^2), which FP32 cannot
-// carry at the small |Q| of force-gradient sub-steps. The 3x3 algebra around
-// them stays in the kernel's own precision.
+// exp(iQ) = f0 + f1 Q + f2 Q^2. On the FP64 path the scalar coefficients
+// are the closed forms of the reference, evaluated from the invariants
+// c0 = Re Tr Q^3 / 3 and c1 = Re Tr Q^2 / 2. The acos argument c0/c0max is
+// clamped to 1: rounding can push it past 1, where the reference formula
+// returns NaN.
 //
-// The acos argument c0/c0max is clamped to 1: rounding can push it past 1,
-// where the reference formula returns NaN.
+// The FP32 molecular-dynamics path cannot use those closed forms: h_j cancels
+// O(1) terms down to O(u^2) and b_ij divide by (9u^2 - w^2)^2 ~ c1^2, so at
+// the small |Q| of force-gradient sub-steps FP32 loses every digit (FP64
+// itself keeps only ~5 digits of b_2j at c1 = 1e-6). There the coefficients
+// come from the exponential series reduced with Cayley-Hamilton
+// (Q^3 = c1 Q + c0 for traceless Hermitian Q): with
+// (iQ)^n / n! = i^n (a_n + b_n Q + d_n Q^2),
+//   a_{n+1} = c0 d_n / (n+1),  b_{n+1} = (a_n + c1 d_n) / (n+1),
+//   d_{n+1} = b_n / (n+1),     (a_0, b_0, d_0) = (1, 0, 0),
+// all real, f_j = sum_n i^n (a_n, b_n, d_n)_j, and b_1j = df_j/dc1,
+// b_2j = df_j/dc0 (hep-lat/0311018 eq. 57) from the recurrence differentiated
+// term by term. 14 terms hold the truncation below 1e-10 for c1 <= 1 (largest
+// eigenvalue |q| <= sqrt(4 c1 / 3)); larger Q fall back to the FP64 closed
+// forms. Both branches keep the reference's exact-zero case (c1 == 0 gives
+// exp(iQ) = 1 and Lambda = 0).
 #pragma once
 #include "su3.hpp"
 
 namespace mag {
 
-struct CHCoef {
+template <class T>
+struct CHC {
   bool zero;
-  cpx<double> f0, f1, f2;
-  cpx<double> b10, b11, b12, b20, b21, b22;
+  cpx<T> f0, f1, f2;
+  cpx<T> b10, b11, b12, b20, b21, b22;
 };
+using CHCoef = CHC<double>;
 
 MAG_HD cpx<double> cexpi(double x) { return {cos(x), sin(x)}; }
 
@@ -103,19 +117,88 @@ MAG_HD CHCoef ch_coefficients(double c0, double c1) {
   return c;
 }
 
-// Invariants of a Hermitian traceless Q, accumulated in FP64.
-template <class T>
-MAG_HD void ch_invariants(const M3<T>& Q, const M3<T>& Q2, double& c0, double& c1) {
-  double t2 = 0., t3 = 0.;
+constexpr int CH_SERIES_TERMS = 14;
+constexpr float CH_SERIES_C1_MAX = 1.0f;
+
+// i^n x added to (re, im), n known at compile time after unrolling.
+MAG_HD void add_phase(int n, float x, float& re, float& im) {
+  switch (n & 3) {
+    case 0: re += x; break;
+    case 1: im += x; break;
+    case 2: re -= x; break;
+    default: im -= x; break;
+  }
+}
+
+// Series coefficients for c1 <= CH_SERIES_C1_MAX (FP32 path).
+template <bool NEED_B>
+MAG_HD CHC<float> ch_series(float c0, float c1) {
+  CHC<float> c;
+  c.zero = (c1 == 0.f);
+  float fr[3] = {1.f, 0.f, 0.f}, fi[3] = {0.f, 0.f, 0.f};
+  float pr[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
+  float qr[3] = {0.f, 0.f, 0.f}, qi[3] = {0.f, 0.f, 0.f};
+  float a = 1.f, b = 0.f, d = 0.f;                  // (iQ)^n / n! in units of i^n
+  float pa = 0.f, pb = 0.f, pd = 0.f;               // d/dc1
+  float qa = 0.f, qb = 0.f, qd = 0.f;               // d/dc0
 #pragma unroll
-  for (int k = 0; k < 3; ++k) t2 += double(Q2.e[4 * k].r);
+  for (int n = 1; n < CH_SERIES_TERMS; ++n) {
+    const float inv = 1.f / float(n);
+    if (NEED_B) {
+      const float npa = c0 * pd * inv, npb = (pa + c1 * pd + d) * inv, npd = pb * inv;
+      const float nqa = (c0 * qd + d) * inv, nqb = (qa + c1 * qd) * inv, nqd = qb * inv;
+      pa = npa; pb = npb; pd = npd;
+      qa = nqa; qb = nqb; qd = nqd;
+      add_phase(n, pa, pr[0], pi[0]); add_phase(n, pb, pr[1], pi[1]); add_phase(n, pd, pr[2], pi[2]);
+      add_phase(n, qa, qr[0], qi[0]); add_phase(n, qb, qr[1], qi[1]); add_phase(n, qd, qr[2], qi[2]);
+    }
+    const float na = c0 * d * inv, nb = (a + c1 * d) * inv, nd = b * inv;
+    a = na; b = nb; d = nd;
+    add_phase(n, a, fr[0], fi[0]); add_phase(n, b, fr[1], fi[1]); add_phase(n, d, fr[2], fi[2]);
+  }
+  if (c.zero) {
+    c.f0 = {1.f, 0.f}; c.f1 = {0.f, 0.f}; c.f2 = {0.f, 0.f};
+    c.b10 = c.b11 = c.b12 = c.b20 = c.b21 = c.b22 = {0.f, 0.f};
+    return c;
+  }
+  c.f0 = {fr[0], fi[0]}; c.f1 = {fr[1], fi[1]}; c.f2 = {fr[2], fi[2]};
+  c.b10 = {pr[0], pi[0]}; c.b11 = {pr[1], pi[1]}; c.b12 = {pr[2], pi[2]};
+  c.b20 = {qr[0], qi[0]}; c.b21 = {qr[1], qi[1]}; c.b22 = {qr[2], qi[2]};
+  return c;
+}
+
+template <class T>
+MAG_HD CHC<T> to_coef(const CHC<double>& c) {
+  CHC<T> r;
+  r.zero = c.zero;
+  r.f0 = {T(c.f0.r), T(c.f0.i)}; r.f1 = {T(c.f1.r), T(c.f1.i)}; r.f2 = {T(c.f2.r), T(c.f2.i)};
+  r.b10 = {T(c.b10.r), T(c.b10.i)}; r.b11 = {T(c.b11.r), T(c.b11.i)}; r.b12 = {T(c.b12.r), T(c.b12.i)};
+  r.b20 = {T(c.b20.r), T(c.b20.i)}; r.b21 = {T(c.b21.r), T(c.b21.i)}; r.b22 = {T(c.b22.r), T(c.b22.i)};
+  return r;
+}
+
+// Invariants of a Hermitian traceless Q in the kernel's precision.
+template <class T>
+MAG_HD void ch_invariants(const M3<T>& Q, const M3<T>& Q2, T& c0, T& c1) {
+  T t2 = T(0), t3 = T(0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) t2 += Q2.e[4 * k].r;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      t3 += double(Q2(i, k).r) * double(Q(k, i).r) - double(Q2(i, k).i) * double(Q(k, i).i);
-  c0 = t3 / 3.;
-  c1 = t2 / 2.;
+    for (int k = 0; k < 3; ++k) t3 += Q2(i, k).r * Q(k, i).r - Q2(i, k).i * Q(k, i).i;
+  c0 = t3 / T(3);
+  c1 = t2 / T(2);
+}
+
+// Coefficients for the kernel's precision: FP64 closed forms, or on the FP32
+// path the series where it converges.
+template <bool NEED_B, class T>
+MAG_HD CHC<T> ch_coef(T c0, T c1) {
+  if constexpr (sizeof(T) == sizeof(float)) {
+    if (c1 <= CH_SERIES_C1_MAX) return ch_series<NEED_B>(c0, c1);
+  }
+  return to_coef<T>(ch_coefficients<NEED_B>(double(c0), double(c1)));
 }
 
 template <class T>
@@ -123,8 +206,7 @@ MAG_HD cpx<T> to(cpx<double> a) { return {T(a.r), T(a.i)}; }
 
 // f0 + f1 Q + f2 Q^2
 template <class T>
-MAG_HD M3<T> ch_combine(const M3<T>& Q, const M3<T>& Q2, cpx<double> f0, cpx<double> f1, cpx<double> f2) {
-  const cpx<T> g0 = to<T>(f0), g1 = to<T>(f1), g2 = to<T>(f2);
+MAG_HD M3<T> ch_combine(const M3<T>& Q, const M3<T>& Q2, cpx<T> g0, cpx<T> g1, cpx<T> g2) {
   M3<T> r;
 #pragma unroll
   for (int k = 0; k < 9; ++k) r.e[k] = cfma(g2, Q2.e[k], g1 * Q.e[k]);
@@ -137,9 +219,9 @@ MAG_HD M3<T> ch_combine(const M3<T>& Q, const M3<T>& Q2, cpx<double> f0, cpx<dou
 template <class T>
 MAG_HD M3<T> expiQ(const M3<T>& Q) {
   const M3<T> Q2 = mul(Q, Q);
-  double c0, c1;
+  T c0, c1;
   ch_invariants(Q, Q2, c0, c1);
-  CHCoef c = ch_coefficients<false>(c0, c1);
+  const CHC<T> c = ch_coef<false>(c0, c1);
   if (c.zero) return unit3<T>();
   return ch_combine(Q, Q2, c.f0, c.f1, c.f2);
 }
@@ -150,9 +232,9 @@ MAG_HD M3<T> expiQ(const M3<T>& Q) {
 template <class T>
 MAG_HD void ch_lambda(const M3<T>& Q, const M3<T>& X, T lam[8], M3<T>& eQ) {
   const M3<T> Q2 = mul(Q, Q);
-  double c0, c1;
+  T c0, c1;
   ch_invariants(Q, Q2, c0, c1);
-  CHCoef c = ch_coefficients<true>(c0, c1);
+  const CHC<T> c = ch_coef<true>(c0, c1);
   if (c.zero) {
 #pragma unroll
     for (int a = 0; a < 8; ++a) lam[a] = T(0);
@@ -164,7 +246,7 @@ MAG_HD void ch_lambda(const M3<T>& Q, const M3<T>& X, T lam[8], M3<T>& eQ) {
   const M3<T> B2 = ch_combine(Q, Q2, c.b20, c.b21, c.b22);
   const cpx<T> t1 = trace_mul(X, B1), t2 = trace_mul(X, B2);
   // Gamma = t1 Q + t2 Q^2 + f1 X + f2 (Q X + X Q)
-  const cpx<T> g1 = to<T>(c.f1), g2 = to<T>(c.f2);
+  const cpx<T> g1 = c.f1, g2 = c.f2;
   M3<T> QX = mul(Q, X);
   mul_acc(QX, X, Q);
   M3<T> G;

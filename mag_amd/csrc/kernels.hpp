@@ -1,10 +1,14 @@
-// gfx950 kernels of the constrained-HMC path. One thread per link (x, mu):
-// blockIdx.x runs over 256-site blocks of one direction and blockIdx.y is the
-// direction mu, so mu and every lattice extent/stride derived from it are
-// wavefront-uniform (scalar registers); only the site coordinates are
-// per-lane. xcd_block() remaps blockIdx.x so that each of the 8 XCDs walks
-// one contiguous slab of the lattice, keeping a link and its stencil
-// neighbours in the same XCD's L2.
+// gfx950 kernels of the constrained-HMC path. One thread per link (x, mu);
+// a block covers 256 consecutive sites of one direction, so mu and every
+// lattice extent/stride derived from it are wavefront-uniform (scalar
+// registers); only the site coordinates are per-lane.
+//
+// Streaming kernels take blockIdx.x = site block, blockIdx.y = mu. Stencil
+// kernels (staples, stout levels) take a 1D grid of 4 * nb tiles, tile =
+// 4 * site block + mu, remapped by stencil_tile() so that each of the 8 XCDs
+// walks one contiguous slab of the lattice and dispatches the four
+// directions of a site block back to back: the neighbour links all four
+// read are then fetched into that XCD's L2 once.
 //
 // Template parameters: TS storage real, NC reals per stored link (12 or 18),
 // TC compute real. FP64 kernels (TS = TC = double, NC = 18) follow the
@@ -21,10 +25,13 @@ namespace mag {
 
 constexpr int BLOCK = 256;
 
-MAG_D int xcd_block() {
+// Blocks are dispatched round-robin over the 8 XCDs (XCD = linear block id
+// mod 8); XCD k gets tiles [k nb/8, (k+1) nb/8) in order.
+MAG_D void stencil_tile(int& mu, int& sblk) {
   const int nb = gridDim.x, b = blockIdx.x;
-  if (nb % 8 != 0) return b;
-  return (b % 8) * (nb / 8) + b / 8;
+  const int t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  mu = t & 3;
+  sblk = t >> 2;
 }
 
 // ======================================================= staples and forces
@@ -167,9 +174,10 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 // 10, and F = 0 for class 100. Stored as F_a = -(beta/3) Im Tr(T_a W).
 template <int ACT, class TS, int NC, class TC>
 MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
-  const int s = xcd_block() * BLOCK + threadIdx.x;
+  int mu, sblk;
+  stencil_tile(mu, sblk);
+  const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
-  const int mu = blockIdx.y;
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
   const TC c0 = TC(1) - TC(8) * c1;
   const Site p = site_of(a.g, s);
@@ -287,9 +295,10 @@ struct StoutFwdArgs {
 
 template <class TSI, int NCI, class TSO, int NCO, class TC>
 MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
-  const int s = xcd_block() * BLOCK + threadIdx.x;
+  int mu, sblk;
+  stencil_tile(mu, sblk);
+  const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
-  const int mu = blockIdx.y;
   const Site p = site_of(a.g, s);
   const M3<TC> S = staple_sum<ACT_WILSON, TSI, NCI, TC>(a.U, a.g, p, mu, TC(1), TC(0));
   const M3<TC> Ux = a.U.template load<TC>(mu, s);
@@ -349,9 +358,10 @@ struct StoutTopArgs {
 
 template <int ACT, class TS, int NC, class TC>
 MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
-  const int s = xcd_block() * BLOCK + threadIdx.x;
+  int mu, sblk;
+  stencil_tile(mu, sblk);
+  const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
-  const int mu = blockIdx.y;
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
   const TC c0 = TC(1) - TC(8) * c1;
   const Site p = site_of(a.g, s);
@@ -462,9 +472,10 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
 
 template <bool LAST, class TS, int NC, class TC>
 MAG_D void k_stout_bwd_body(const StoutBwdArgs<TS, NC>& a) {
-  const int s = xcd_block() * BLOCK + threadIdx.x;
+  int mu, sblk;
+  stencil_tile(mu, sblk);
+  const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
-  const int mu = blockIdx.y;
   const Site p = site_of(a.g, s);
   const M3<TC> Z = stout_six_terms<TS, NC, TC>(a, p, mu);
   if (!LAST) {

@@ -55,9 +55,9 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   if (act.n_stout <= 0) {
     if (iw)
       return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, TS, NC, TS>,
-                    k_gauge_force_b<ACT_IWASAKI, TS, NC, TS>, grid_links(g), fa);
+                    k_gauge_force_b<ACT_IWASAKI, TS, NC, TS>, grid_stencil(g), fa);
     return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, TS, NC, TS>, k_gauge_force_b<ACT_WILSON, TS, NC, TS>,
-                  grid_links(g), fa);
+                  grid_stencil(g), fa);
   }
   const int n = act.n_stout;
   if ((r = ws->need_stout(n))) return r;
@@ -65,7 +65,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   for (int k = 0; k < n; ++k) {
     StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g), mview<TS>(ws->Cs[k], g),
                                     lview<TS, NC>(ws->Us[k], g), act.rho, 1};
-    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_links(g),
+    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_stencil(g),
                     fw)))
       return r;
   }
@@ -83,10 +83,10 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
                             act.rho};
     if (iw)
       r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS>, k_stout_top_b<ACT_IWASAKI, TS, NC, TS>,
-                 grid_links(g), tp);
+                 grid_stencil(g), tp);
     else
       r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS>, k_stout_top_b<ACT_WILSON, TS, NC, TS>,
-                 grid_links(g), tp);
+                 grid_stencil(g), tp);
     if (r) return r;
   }
   for (int k = n - 1; k >= 1; --k) {
@@ -101,7 +101,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     bw.Lout = aview<TS>(ws->L[1 - cur], g);
     bw.Wout = mview<TS>(ws->W, g);
     bw.rho = act.rho;
-    if ((r = launch(c, "stout_bwd", k_stout_bwd<false, TS, NC, TS>, k_stout_bwd_b<false, TS, NC, TS>, grid_links(g),
+    if ((r = launch(c, "stout_bwd", k_stout_bwd<false, TS, NC, TS>, k_stout_bwd_b<false, TS, NC, TS>, grid_stencil(g),
                     bw)))
       return r;
     cur = 1 - cur;
@@ -117,7 +117,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     if ((r = ws->need_Z())) return r;
     bw.Zout = mview<TS>(ws->Z, g);
   }
-  if ((r = launch(c, "stout_bwd_force", k_stout_bwd<true, TS, NC, TS>, k_stout_bwd_b<true, TS, NC, TS>, grid_links(g),
+  if ((r = launch(c, "stout_bwd_force", k_stout_bwd<true, TS, NC, TS>, k_stout_bwd_b<true, TS, NC, TS>, grid_stencil(g),
                   bw)))
     return r;
   if (!act.mag) return MAG_OK;
@@ -199,13 +199,13 @@ int Ops<TS, NC>::smeared_action_sums(mag_ctx c, const Geom& g, void* U, double r
       StoutFwdArgs<TS, NC, double, 18> fw{g, lview<TS, NC>(U, g), aview<double>(nullptr, g),
                                           mview<double>(nullptr, g), lview<double, 18>(dst, g), rho, 0};
       r = launch(c, "stout_fwd_h", k_stout_fwd<TS, NC, double, 18, double>, k_stout_fwd_b<TS, NC, double, 18, double>,
-                 grid_links(g), fw);
+                 grid_stencil(g), fw);
     } else {
       StoutFwdArgs<double, 18, double, 18> fw{g, lview<double, 18>(wsd->Uh[(k - 1) & 1], g),
                                               aview<double>(nullptr, g), mview<double>(nullptr, g),
                                               lview<double, 18>(dst, g), rho, 0};
       r = launch(c, "stout_fwd_h", k_stout_fwd<double, 18, double, 18, double>,
-                 k_stout_fwd_b<double, 18, double, 18, double>, grid_links(g), fw);
+                 k_stout_fwd_b<double, 18, double, 18, double>, grid_stencil(g), fw);
     }
     if (r) return r;
   }
@@ -228,7 +228,7 @@ int Ops<TS, NC>::smear(mag_ctx c, const Geom& g, void* Uout, void* U, double rho
     if (dst == src) dst = (src == Uout) ? ws->Uaux : Uout;
     StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(src, g), aview<TS>(nullptr, g), mview<TS>(nullptr, g),
                                     lview<TS, NC>(dst, g), rho, 0};
-    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_links(g),
+    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_stencil(g),
                     fw)))
       return r;
     src = dst;

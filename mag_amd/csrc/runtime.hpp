@@ -167,6 +167,8 @@ inline int fetch_scalars(mag_ctx c, int n) {
 }
 
 inline dim3 grid_links(const Geom& g) { return dim3((g.V + RT_BLOCK - 1) / RT_BLOCK, 4, 1); }
+// Stencil kernels: one 1D tile per (256-site block, mu), see stencil_tile().
+inline dim3 grid_stencil(const Geom& g) { return dim3(4 * ((g.V + RT_BLOCK - 1) / RT_BLOCK), 1, 1); }
 
 // ------------------------------------------------------------ workspace
 // Scratch of one lattice shape and storage format, owned by the context and

@@ -1,30 +1,35 @@
 // Continuous-batching HMC server. Independent chains (each with its own
 // volume, action, beta, smearing, precision, seed and trajectory count)
-// are submitted at any time. Every step():
-//   1. admits pending chains in arrival order while their device footprint
-//      fits the HBM budget (links, momenta, backup, workspace and RNG state
-//      then stay resident until the chain retires);
-//   2. groups running chains into shape classes (volume, precision, action
-//      kind, smearing depth, constraint, integrator, MD steps) and splits
-//      each class into batches of at most max_batch chains;
-//   3. runs one trajectory of every chain: a batch's trajectory is recorded
-//      once per chain through the launch layer and emitted as one launch per
-//      kernel over all its chains (blockIdx.z = chain), optionally captured
-//      in a HIP graph and replayed while the batch membership is unchanged;
-//      batches go to a pool of streams, so batches of different classes
-//      overlap on the device;
-//   4. gathers each chain's FP64 scalars with one copy per batch, makes the
-//      accept decisions on the host, restores rejected chains, records
-//      per-trajectory completion times, retires finished, cancelled and
-//      non-finite chains (freeing their memory for the next admissions).
+// are submitted at any time and run asynchronously:
+//   - admission: pending chains are admitted in arrival order while their
+//     device footprint fits the HBM budget (links, momenta, backup,
+//     workspace and RNG state then stay resident until the chain retires);
+//   - batches: chains that are not on the device are grouped into shape
+//     classes (volume, precision, action kind, smearing depth, constraint,
+//     integrator, MD steps) and batches of at most max_batch chains. A
+//     batch's trajectory is recorded once per chain through the launch layer
+//     and emitted as one launch per kernel over all its chains
+//     (blockIdx.z = chain), optionally captured in a HIP graph that is
+//     replayed while the membership is unchanged;
+//   - overlap: every batch is in flight on its own schedule. It goes to the
+//     least-loaded of a pool of streams, and its completion is an event, so
+//     a batch of small lattices runs its next trajectory as soon as its last
+//     one finished, without waiting for batches of large lattices;
+//   - completion: a finished batch's FP64 scalars (one copy per batch) feed
+//     the accept decisions on the host; rejected chains are restored on the
+//     batch's stream; finished, cancelled and non-finite chains retire, and
+//     their memory returns to the budget once the device is done with it.
+// mag_server_step() admits, launches every idle batch, then waits until at
+// least one batch in flight has completed and processes all that have.
 // A chain's kernels and reductions are the same alone or in a batch, so its
-// results do not depend on what it is batched with.
+// results do not depend on what it is batched with or when.
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -39,6 +44,8 @@ constexpr int NSTREAMS = 4;
 
 enum { ST_PENDING = 0, ST_RUNNING = 1, ST_DONE = 2, ST_FAILED = 3, ST_CANCELLED = 4 };
 
+struct Batch;
+
 struct Entry {
   int id = 0;
   mag_serve_request req{};
@@ -48,6 +55,7 @@ struct Entry {
   std::vector<mag_traj_result> results;
   std::vector<double> t_done;
   bool cancel = false;
+  Batch* batch = nullptr;  // batch the chain belongs to (null: not yet batched)
 };
 
 using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
@@ -57,17 +65,35 @@ ClassKey class_of(const mag_serve_request& r) {
                   r.act.n_stout, r.act.mag, r.hmc.integrator, r.hmc.trajectory_length, r.hmc.reunitarize ? 1 : 0);
 }
 
+// Relative device work of one trajectory of one chain (for stream balancing).
+double traj_cost(const mag_serve_request& r) {
+  const double V = (double)r.dims[0] * r.dims[1] * r.dims[2] * r.dims[3];
+  const double per_force = (r.act.type == MAG_IWASAKI ? 4. : 1.) + 3. * r.act.n_stout;
+  return V * (r.hmc.trajectory_length * (r.hmc.integrator == 0 ? 4. : 1.) * per_force + 4.);
+}
+
 struct Batch {
+  ClassKey key;
   std::vector<Entry*> members;
-  hipStream_t stream = nullptr;
+  int sidx = 0;
+  double cost = 0.;
   Recorder rec;
   void* tables = nullptr;
   size_t table_bytes = 0;
   double* gather = nullptr;  // S_COUNT doubles per member (device)
+  size_t gather_n = 0;
   double* h_gather = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  bool valid = false;
+  hipEvent_t done = nullptr;
+  bool valid = false;     // recorded tables / graph match the members
+  bool inflight = false;
+};
+
+struct Grave {
+  mag_chain chain;
+  hipEvent_t ev;
+  size_t footprint;
 };
 
 }  // namespace
@@ -78,9 +104,12 @@ struct mag_server_s {
   int max_batch = 64;
   bool graphs = true;
   hipStream_t streams[NSTREAMS] = {};
+  hipStream_t aux = nullptr;  // admission, table uploads, graph capture
+  double load[NSTREAMS] = {};
   std::vector<std::unique_ptr<Entry>> entries;
   std::deque<Entry*> pending;
-  std::map<std::pair<ClassKey, int>, std::unique_ptr<Batch>> batches;
+  std::vector<std::unique_ptr<Batch>> batches;
+  std::vector<Grave> graves;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   long trajectories = 0;
 };
@@ -91,35 +120,45 @@ double now_s(mag_server s) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
 }
 
-void free_batch(mag_server s, Batch* b) {
+void release_tables(mag_server s, Batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
   if (b->tables) dev_free(s->ctx, b->tables, b->table_bytes);
-  if (b->gather) dev_free(s->ctx, b->gather, b->members.size() * S_COUNT * sizeof(double));
+  if (b->gather) dev_free(s->ctx, b->gather, b->gather_n * S_COUNT * sizeof(double));
   if (b->h_gather) (void)hipHostFree(b->h_gather);
   b->tables = nullptr;
   b->gather = nullptr;
+  b->gather_n = 0;
   b->h_gather = nullptr;
   b->rec = Recorder();
   b->valid = false;
 }
 
+void destroy_batch(mag_server s, Batch* b) {
+  release_tables(s, b);
+  if (b->done) (void)hipEventDestroy(b->done);
+  b->done = nullptr;
+}
+
 // Records the trajectory of every member, uploads the argument tables and
-// (optionally) captures the emitted launches in a graph.
+// (optionally) captures the emitted launches in a graph, all on the
+// auxiliary stream, which is then synchronised (no batch in flight waits).
 int build_batch(mag_server s, Batch* b) {
   mag_ctx x = s->ctx;
   const size_t n = b->members.size();
   int r;
-  free_batch(s, b);
+  release_tables(s, b);
+  if (!b->done && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
+    return fail(x, MAG_ERR_DEVICE, "event creation failed");
   if ((r = dev_alloc(x, (void**)&b->gather, n * S_COUNT * sizeof(double)))) return r;
+  b->gather_n = n;
   if (hipHostMalloc(&b->h_gather, n * S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess)
     return fail(x, MAG_ERR_OOM, "pinned gather buffer");
   b->rec = Recorder();
   for (size_t i = 0; i < n; ++i) {
     mag_chain c = b->members[i]->chain;
-    c->ctx->stream = b->stream;
     b->rec.begin_chain((int)i);
     c->ctx->rec = &b->rec;
     r = chain_traj_kernels(c, true);
@@ -141,15 +180,15 @@ int build_batch(mag_server s, Batch* b) {
     c.table = (char*)b->tables + off;
     off += (c.blob.size() + 255) / 256 * 256;
   }
-  HIPCHK(x, hipMemcpyAsync(b->tables, host.data(), total, hipMemcpyHostToDevice, b->stream));
-  HIPCHK(x, hipStreamSynchronize(b->stream));
+  HIPCHK(x, hipMemcpyAsync(b->tables, host.data(), total, hipMemcpyHostToDevice, s->aux));
+  HIPCHK(x, hipStreamSynchronize(s->aux));
   if (s->graphs) {
-    HIPCHK(x, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    HIPCHK(x, hipStreamBeginCapture(s->aux, hipStreamCaptureModeThreadLocal));
     for (auto& c : b->rec.calls) {
       void* args[] = {&c.table};
-      (void)hipLaunchKernel(c.kb, dim3(c.grid.x, c.grid.y, (unsigned)n), dim3(BLOCK), args, 0, b->stream);
+      (void)hipLaunchKernel(c.kb, dim3(c.grid.x, c.grid.y, (unsigned)n), dim3(BLOCK), args, 0, s->aux);
     }
-    HIPCHK(x, hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(x, hipStreamEndCapture(s->aux, &b->graph));
     HIPCHK(x, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   b->valid = true;
@@ -159,35 +198,81 @@ int build_batch(mag_server s, Batch* b) {
 int launch_batch(mag_server s, Batch* b) {
   mag_ctx x = s->ctx;
   const size_t n = b->members.size();
+  int best = 0;
+  for (int i = 1; i < NSTREAMS; ++i)
+    if (s->load[i] < s->load[best]) best = i;
+  b->sidx = best;
+  hipStream_t st = s->streams[best];
+  for (Entry* e : b->members) e->chain->ctx->stream = st;
   // trajectory index of every member (read by the momentum kernel)
   for (size_t i = 0; i < n; ++i) {
     mag_chain c = b->members[i]->chain;
     uint64_t e = (uint64_t)c->traj;
-    HIPCHK(x, hipMemcpyAsync(c->d_epoch, &e, sizeof e, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(x, hipMemcpyAsync(c->d_epoch, &e, sizeof e, hipMemcpyHostToDevice, st));
   }
   if (b->exec) {
-    HIPCHK(x, hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(x, hipGraphLaunch(b->exec, st));
   } else {
     for (auto& c : b->rec.calls) {
       void* args[] = {&c.table};
-      HIPCHK(x, hipLaunchKernel(c.kb, dim3(c.grid.x, c.grid.y, (unsigned)n), dim3(BLOCK), args, 0, b->stream));
+      HIPCHK(x, hipLaunchKernel(c.kb, dim3(c.grid.x, c.grid.y, (unsigned)n), dim3(BLOCK), args, 0, st));
     }
   }
-  HIPCHK(x, hipMemcpyAsync(b->h_gather, b->gather, n * S_COUNT * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(x, hipMemcpyAsync(b->h_gather, b->gather, n * S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(x, hipEventRecord(b->done, st));
+  b->cost = 0.;
+  for (Entry* e : b->members) b->cost += traj_cost(e->req);
+  s->load[best] += b->cost;
+  b->inflight = true;
   return MAG_OK;
 }
 
-void retire(mag_server s, Entry* e, int state) {
+// Retires a chain: its memory is released once the work already queued on
+// its stream (a restore copy, say) has finished.
+int retire(mag_server s, Entry* e, int state) {
   if (e->chain) {
-    (void)hipStreamSynchronize(e->chain->ctx->stream);
-    mag_chain_destroy(e->chain);
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(ev, e->chain->ctx->stream) != hipSuccess)
+      return fail(s->ctx, MAG_ERR_DEVICE, "retire event failed");
+    s->graves.push_back({e->chain, ev, e->footprint});
     e->chain = nullptr;
+  } else {
+    s->used -= e->footprint;
   }
-  s->used -= e->footprint;
+  if (e->batch) {
+    auto& m = e->batch->members;
+    for (size_t i = 0; i < m.size(); ++i)
+      if (m[i] == e) {
+        m.erase(m.begin() + (long)i);
+        break;
+      }
+    e->batch->valid = false;
+    e->batch = nullptr;
+  }
   e->state = state;
+  return MAG_OK;
+}
+
+int reap(mag_server s, bool wait) {
+  std::vector<Grave> keep;
+  for (Grave& g : s->graves) {
+    const hipError_t q = wait ? hipEventSynchronize(g.ev) : hipEventQuery(g.ev);
+    if (q == hipErrorNotReady) {
+      keep.push_back(g);
+      continue;
+    }
+    if (q != hipSuccess) return fail(s->ctx, MAG_ERR_DEVICE, "retired chain's stream failed");
+    (void)hipEventDestroy(g.ev);
+    mag_chain_destroy(g.chain);
+    s->used -= g.footprint;
+  }
+  s->graves.swap(keep);
+  return MAG_OK;
 }
 
 int admit(mag_server s) {
+  bool any = false;
   while (!s->pending.empty()) {
     Entry* e = s->pending.front();
     if (e->cancel) {
@@ -198,7 +283,7 @@ int admit(mag_server s) {
     if (s->used + e->footprint > s->budget) break;  // arrival order: wait for memory
     s->pending.pop_front();
     mag_ctx cc = nullptr;
-    int r = ctx_child_create(s->ctx, s->streams[0], &cc);
+    int r = ctx_child_create(s->ctx, s->aux, &cc);
     if (r) return r;
     mag_chain ch = nullptr;
     r = mag_chain_create(cc, e->req.dims, &e->req.act, &e->req.hmc, &ch);
@@ -214,10 +299,122 @@ int admit(mag_server s) {
     if (e->req.hmc.accept_mode != 0 && (r = chain_init_demon(ch))) return r;
     s->used += e->footprint;
     e->state = ST_RUNNING;
+    any = true;
   }
-  // admission work ran on stream 0; batches may run on any stream
-  HIPCHK(s->ctx, hipStreamSynchronize(s->streams[0]));
+  if (any) HIPCHK(s->ctx, hipStreamSynchronize(s->aux));
   return MAG_OK;
+}
+
+// Accept steps of a completed batch, results, retirement.
+int complete(mag_server s, Batch* b) {
+  const double t = now_s(s);
+  s->load[b->sidx] -= b->cost;
+  b->inflight = false;
+  const std::vector<Entry*> mem = b->members;
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Entry* e = mem[i];
+    mag_traj_result res{};
+    int r;
+    if ((r = chain_accept(e->chain, b->h_gather + i * S_COUNT, &res))) return r;
+    e->results.push_back(res);
+    e->t_done.push_back(t);
+    ++s->trajectories;
+    int st = -1;
+    if (res.status != MAG_OK) st = ST_FAILED;
+    else if (e->cancel) st = ST_CANCELLED;
+    else if ((int)e->results.size() >= e->req.ntraj) st = ST_DONE;
+    if (st >= 0 && (r = retire(s, e, st))) return r;
+  }
+  return MAG_OK;
+}
+
+// Regroups the chains that are not on the device and launches every idle
+// batch. Idle batches of one class are merged up to max_batch; chains not
+// yet batched join the class's idle batches or open new ones.
+int launch_idle(mag_server s) {
+  int r;
+  for (auto& e : s->entries)
+    if (e->state == ST_RUNNING && e->cancel && !(e->batch && e->batch->inflight) &&
+        (r = retire(s, e.get(), ST_CANCELLED)))
+      return r;
+  std::map<ClassKey, std::vector<Batch*>> idle;
+  for (auto& b : s->batches)
+    if (!b->inflight && !b->members.empty()) idle[b->key].push_back(b.get());
+  std::map<ClassKey, std::vector<Entry*>> loose;
+  for (auto& e : s->entries)
+    if (e->state == ST_RUNNING && !e->batch) loose[class_of(e->req)].push_back(e.get());
+  for (auto& kv : loose) {
+    auto& bs = idle[kv.first];
+    for (Entry* e : kv.second) {
+      Batch* home = nullptr;
+      for (Batch* b : bs)
+        if ((int)b->members.size() < s->max_batch) {
+          home = b;
+          break;
+        }
+      if (!home) {
+        s->batches.push_back(std::make_unique<Batch>());
+        home = s->batches.back().get();
+        home->key = kv.first;
+        bs.push_back(home);
+      }
+      home->members.push_back(e);
+      home->valid = false;
+      e->batch = home;
+    }
+  }
+  for (auto& kv : idle) {
+    auto& bs = kv.second;
+    for (size_t i = 0; i < bs.size(); ++i)
+      for (size_t j = i + 1; j < bs.size(); ++j)
+        if (!bs[j]->members.empty() && bs[i]->members.size() + bs[j]->members.size() <= (size_t)s->max_batch) {
+          for (Entry* e : bs[j]->members) {
+            bs[i]->members.push_back(e);
+            e->batch = bs[i];
+          }
+          bs[j]->members.clear();
+          bs[i]->valid = false;
+        }
+  }
+  // drop empty idle batches, launch the rest
+  std::vector<std::unique_ptr<Batch>> keep;
+  for (auto& b : s->batches) {
+    if (!b->inflight && b->members.empty()) {
+      destroy_batch(s, b.get());
+      continue;
+    }
+    keep.push_back(std::move(b));
+  }
+  s->batches.swap(keep);
+  for (auto& b : s->batches) {
+    if (b->inflight) continue;
+    if (!b->valid && (r = build_batch(s, b.get()))) return r;
+    if ((r = launch_batch(s, b.get()))) return r;
+  }
+  return MAG_OK;
+}
+
+// Waits until at least one batch in flight has completed (if any is), then
+// completes every batch that has.
+int poll(mag_server s) {
+  for (;;) {
+    bool inflight = false, any = false;
+    for (size_t i = 0; i < s->batches.size(); ++i) {
+      Batch* b = s->batches[i].get();
+      if (!b->inflight) continue;
+      const hipError_t q = hipEventQuery(b->done);
+      if (q == hipErrorNotReady) {
+        inflight = true;
+        continue;
+      }
+      if (q != hipSuccess) return fail(s->ctx, MAG_ERR_DEVICE, "batch failed on the device");
+      int r;
+      if ((r = complete(s, b))) return r;
+      any = true;
+    }
+    if (any || !inflight) return MAG_OK;
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
 }
 
 }  // namespace
@@ -242,6 +439,10 @@ int mag_server_create(mag_ctx ctx, size_t hbm_budget, int max_batch, int use_gra
       mag_server_destroy(s);
       return fail(ctx, MAG_ERR_DEVICE, "stream creation failed");
     }
+  if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess) {
+    mag_server_destroy(s);
+    return fail(ctx, MAG_ERR_DEVICE, "stream creation failed");
+  }
   *out = s;
   return MAG_OK;
 }
@@ -250,11 +451,16 @@ int mag_server_destroy(mag_server s) {
   if (!s) return MAG_ERR_ARG;
   for (auto& st : s->streams)
     if (st) (void)hipStreamSynchronize(st);
-  for (auto& kv : s->batches) free_batch(s, kv.second.get());
-  for (auto& e : s->entries)
-    if (e->chain) retire(s, e.get(), ST_CANCELLED);
+  if (s->aux) (void)hipStreamSynchronize(s->aux);
+  for (auto& b : s->batches) destroy_batch(s, b.get());
+  for (auto& e : s->entries) {
+    e->batch = nullptr;
+    if (e->chain) (void)retire(s, e.get(), ST_CANCELLED);
+  }
+  (void)reap(s, true);
   for (auto& st : s->streams)
     if (st) (void)hipStreamDestroy(st);
+  if (s->aux) (void)hipStreamDestroy(s->aux);
   delete s;
   return MAG_OK;
 }
@@ -286,73 +492,13 @@ int mag_server_cancel(mag_server s, int id) {
 int mag_server_step(mag_server s, int* active, int* pending) {
   if (!s) return MAG_ERR_ARG;
   int r;
-  if ((r = admit(s))) return r;
-  // group running chains into shape classes and batches
-  std::map<ClassKey, std::vector<Entry*>> classes;
-  for (auto& e : s->entries)
-    if (e->state == ST_RUNNING) classes[class_of(e->req)].push_back(e.get());
-  std::map<std::pair<ClassKey, int>, std::unique_ptr<Batch>> next;
-  int si = 0;
-  std::vector<Batch*> order;
-  for (auto& kv : classes) {
-    const auto& v = kv.second;
-    for (size_t b0 = 0, bi = 0; b0 < v.size(); b0 += s->max_batch, ++bi) {
-      std::vector<Entry*> mem(v.begin() + b0, v.begin() + std::min(v.size(), b0 + (size_t)s->max_batch));
-      auto key = std::make_pair(kv.first, (int)bi);
-      auto it = s->batches.find(key);
-      std::unique_ptr<Batch> b;
-      if (it != s->batches.end() && it->second->members == mem) {
-        b = std::move(it->second);
-      } else {
-        if (it != s->batches.end()) free_batch(s, it->second.get());
-        b = std::make_unique<Batch>();
-        b->members = mem;
-      }
-      b->stream = s->streams[si++ % NSTREAMS];
-      order.push_back(b.get());
-      next[key] = std::move(b);
-    }
-  }
-  for (auto& kv : s->batches)
-    if (kv.second) free_batch(s, kv.second.get());
-  s->batches = std::move(next);
-  // launch every batch (streams overlap), then wait
-  for (Batch* b : order) {
-    for (Entry* e : b->members) e->chain->ctx->stream = b->stream;
-    if (!b->valid && (r = build_batch(s, b))) return r;
-    if ((r = launch_batch(s, b))) return r;
-  }
-  for (auto& st : s->streams) HIPCHK(s->ctx, hipStreamSynchronize(st));
-  const double t = now_s(s);
-  // accept steps, results, retirement
-  for (Batch* b : order) {
-    bool changed = false;
-    for (size_t i = 0; i < b->members.size(); ++i) {
-      Entry* e = b->members[i];
-      mag_traj_result res{};
-      if ((r = chain_accept(e->chain, b->h_gather + i * S_COUNT, &res))) return r;
-      e->results.push_back(res);
-      e->t_done.push_back(t);
-      ++s->trajectories;
-      if (res.status != MAG_OK) {
-        retire(s, e, ST_FAILED);
-        changed = true;
-      } else if (e->cancel) {
-        retire(s, e, ST_CANCELLED);
-        changed = true;
-      } else if ((int)e->results.size() >= e->req.ntraj) {
-        retire(s, e, ST_DONE);
-        changed = true;
-      }
-    }
-    if (changed) b->valid = false;
-  }
-  for (auto& st : s->streams) HIPCHK(s->ctx, hipStreamSynchronize(st));
+  if ((r = reap(s, false)) || (r = admit(s)) || (r = launch_idle(s)) || (r = poll(s))) return r;
   int na = 0, np = 0;
   for (auto& e : s->entries) {
     na += e->state == ST_RUNNING;
     np += e->state == ST_PENDING;
   }
+  if (na == 0 && np == 0 && (r = reap(s, true))) return r;
   if (active) *active = na;
   if (pending) *pending = np;
   return MAG_OK;
