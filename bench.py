@@ -41,10 +41,10 @@ M_FLOPS = 198  # one 3x3 complex matrix product
 # momentum update. The Lambda step is 1/6 of a backward level (10M), the
 # six-term sum 5/6 (50M), in proportion to the products each executes.
 KERNEL_UNITS = {
-    # family: (bytes per link, 3x3 products per link)
+    # family: (bytes per link, 3x3 products per link); links 12 reals (48 B)
     "stout_fwd": (48 + 48 + 32 + 72, 15),          # read U_k; write U_k+1, Q_k, C_k
     "stout_top": (48 + 48 + 32 + 72 + 32 + 72, 85 + 10),   # read U_n, U_n-1, Q, C; write Lambda, W
-    "stout_bwd": (48 + 32 + 72 + 48 + 32 + 72 + 32 + 72, 60),  # read U_k, Lambda_k, W_k, U, Q, C_k-1; write Lambda, W
+    "stout_bwd": (48 + 32 + 72 + 48 + 32 + 72 + 32 + 72, 60),  # U_k, Lambda_k, W_k, U_k-1, Q, C; Lambda, W
     "stout_bwd_force": (48 + 32 + 72 + 32 + 32, 50),  # read U_0, Lambda_0, W_0, P; write P
     "link_update": (48 + 32 + 48, 3),
     "force_iwasaki": (48 + 32 + 32, 85),
@@ -53,6 +53,8 @@ KERNEL_UNITS = {
 # FP32 12-real thin links (the numerator of "force GB/s").
 FORCE_BYTES_LINK_C3 = 1520
 FORCE_PRODUCTS_LINK = {0: 85, 1: 160, 2: 235, 3: 310}
+# action: (kind, default beta, c1); c0 = 1 - 8 c1
+ACTIONS = {"iwasaki": (1, 2.6, -0.331), "dbw2": (1, 1.04, -1.4069), "wilson": (0, 6.0, 0.0)}
 
 
 def parse():
@@ -70,6 +72,10 @@ def parse():
                    help="chains of the serving sample (0: skip); the full stream has 512")
     p.add_argument("--serve-max-traj", type=int, default=10, help="trajectory cap per chain in the sample")
     p.add_argument("--seed", type=int, default=32)
+    p.add_argument("--action", choices=sorted(ACTIONS), default="iwasaki",
+                   help="gauge action (dbw2 with --L 64 --n-md 16 --wloop is the 64^4 DBW2 workload)")
+    p.add_argument("--beta", type=float, default=None, help="default: 2.6 Iwasaki, 1.04 DBW2, 6.0 Wilson")
+    p.add_argument("--wloop", action="store_true", help="measure a 4x4 Wilson loop after every trajectory")
     p.add_argument("--no-reversibility", dest="reversibility", action="store_false",
                    help="skip the FP64 reversibility round trip after the timed run")
     return p.parse_args()
@@ -145,10 +151,12 @@ def thermalize(M, ctx, L, arg, seed, ntraj):
     return ch
 
 
-def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing):
+def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing, wloop=False):
     """A fresh chain (trajectory 0, Metropolis + demon window of 0.5% of the
-    initial plaquette sum) from the links of `start`: `steps` trajectories,
-    each timed on its own. Returns (seconds per trajectory, results, chain)."""
+    initial plaquette sum) from the links of `start`: `steps` trajectories
+    timed together between two device synchronisations (with `wloop`, one
+    r = (4,0,0), T = 4 Wilson loop after each trajectory, inside the timed
+    region). Returns (seconds, results, chain, Wilson loops)."""
     ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005)
     ch.links().copy_from(start.links())
     ch.demon_state()  # window centre from the starting links, outside the timed region
@@ -156,12 +164,19 @@ def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing):
     if timing:
         ctx.reset_timing()
         ctx.set_timing(True)
+    loops = []
     t0 = time.perf_counter()
-    res = ch.run(steps)
+    if wloop:
+        res = []
+        for _ in range(steps):
+            res += ch.run(1)
+            loops.append(M.avg_wilson_loop(ch.links(), (4, 0, 0), 4))
+    else:
+        res = ch.run(steps)
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
-    return dt, res, ch
+    return dt, res, ch, loops
 
 
 def main():
@@ -174,14 +189,17 @@ def main():
     import mag_amd as M
     ctx = M.Context(0)
     L, n = a.L, a.n_md
-    arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=2.6, gauge=M.Gauge(M.IWASAKI, -0.331), n_stout=a.n_stout,
+    kind, beta, c1 = ACTIONS[a.action]
+    beta = a.beta if a.beta is not None else beta
+    arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=beta, gauge=M.Gauge(kind, c1), n_stout=a.n_stout,
                      rho=0.1, num_forced_accept_step=0)
+    act_name = {"iwasaki": "Iwasaki", "dbw2": "DBW2", "wilson": "Wilson"}[a.action]
     V = L ** 4
     nlinks = 4 * V
-    print(f"bench: {L}^4 Iwasaki+{a.n_stout}stout FP32-MD, warmup {a.warmup}, steps {a.steps}", file=sys.stderr,
+    print(f"bench: {L}^4 {act_name}+{a.n_stout}stout FP32-MD, warmup {a.warmup}, steps {a.steps}", file=sys.stderr,
           flush=True)
     therm = thermalize(M, ctx, L, arg, a.seed, a.warmup)
-    dt, res, ch32 = timed_chain(M, ctx, L, arg, a.seed, M.F32, therm, a.steps, True)
+    dt, res, ch32, loops = timed_chain(M, ctx, L, arg, a.seed, M.F32, therm, a.steps, True, a.wloop)
     print(f"bench: FP32 {a.steps} trajectories in {dt:.3f} s", file=sys.stderr, flush=True)
     fams = {}
     for f in list(KERNEL_UNITS) + ["stout_fwd_h", "action", "kinetic", "reduce", "mom_refresh", "reunitarize"]:
@@ -216,8 +234,9 @@ def main():
            "value": round(traj_s, 4), "unit": "traj/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (Haar-random links, seed %d)" % a.seed,
-           "config": {"workload": f"{L}^4 Iwasaki(c1=-0.331, beta=2.6) + {a.n_stout}-stout(rho=0.1), "
-                                  f"FG tau=1 n={n}, FP32 12-real links, FP64 dH, demon window 0.5%",
+           "config": {"workload": f"{L}^4 {act_name}(c1={c1}, beta={beta}) + {a.n_stout}-stout(rho=0.1), "
+                                  f"FG tau=1 n={n}, FP32 12-real links, FP64 dH, demon window 0.5%"
+                                  + (", 4x4 Wilson loop per trajectory" if a.wloop else ""),
                       "lattice": [L] * 4, "parallelism": "single GPU"},
            "link_updates_per_s": round(4 * V * n * traj_s, 1),
            "force_gbs": round(force_gbs, 1) if force_gbs else None,
@@ -226,8 +245,10 @@ def main():
            "dh": [round(r.dh, 5) for r in res],
            "kernel_ms_per_traj": {f: round(v[0] / a.steps, 3) for f, v in fams.items()},
            "roofline": roof}
+    if loops:
+        out["wilson_loop_4x4"] = [round(w, 8) for w in loops]
     if a.fp64_steps > 0:
-        dt64, r64, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.fp64_steps, False)
+        dt64, r64, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.fp64_steps, False, a.wloop)
         out["fp64_traj_per_s"] = round(a.fp64_steps / dt64, 4)
         out["fp32_over_fp64"] = round(traj_s / (a.fp64_steps / dt64), 3)
         out["fp64_accept_rate"] = round(sum(r.accept for r in r64) / len(r64), 3)

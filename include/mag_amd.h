@@ -44,7 +44,9 @@ enum {
   MAG_ERR_NAN = -5,         /* non-finite Hamiltonian: trajectory rejected */
   MAG_ERR_UNSUPPORTED = -6, /* combination not implemented */
   MAG_ERR_BUDGET = -7,      /* chain does not fit the HBM budget */
-  MAG_ERR_STATE = -8        /* call not valid in the object's current state */
+  MAG_ERR_STATE = -8,       /* call not valid in the object's current state */
+  MAG_ERR_IO = -9,          /* file missing, unreadable or not in the expected format */
+  MAG_ERR_CHECKSUM = -10    /* stored checksum or plaquette does not match the data */
 };
 
 /* field kinds and precisions */
@@ -181,14 +183,49 @@ int mag_chain_traj_index(mag_chain c, long* next_traj);
 int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
                              long acc_rej[2]);
 
+/* ---------------------------------------------------------- NERSC config I/O
+ * export_config_nersc / import_config_nersc (field-matrix.h:445-713): text
+ * header BEGIN_HEADER ... END_HEADER, then the links in site order
+ * (x fastest), mu 0..3 per site, row-major complex IEEE64 big-endian; 18
+ * reals per link (DATATYPE 4D_SU3_GAUGE_3x3) or the first two rows, 12 reals
+ * (4D_SU3_GAUGE, third row rebuilt by reunitarize on import). CHECKSUM is
+ * the 32-bit sum of the data taken as 32-bit words. */
+typedef struct {
+  int dims[4];
+  int skip_third;            /* 1: 4D_SU3_GAUGE (two rows); 0: 4D_SU3_GAUGE_3x3 */
+  double link_trace;         /* LINK_TRACE: average Re Tr U / 3 */
+  double plaquette;          /* PLAQUETTE: avg_plaquette */
+  double beta;               /* BETA */
+  int sequence_number;       /* SEQUENCE_NUMBER */
+  char ensemble_label[128];  /* ENSEMBLE_LABEL */
+  uint32_t checksum;         /* CHECKSUM */
+  uint32_t crc32;            /* CRC32HASH: CRC-32 (IEEE) of the data section as stored */
+} mag_nersc_header;
+
+/* Host-side file I/O of links in the qlat AoS layout [site][mu][3][3][re,im]
+ * (no device work). Write fills h->checksum and h->crc32; read verifies the
+ * checksum (aos may be NULL to read the header only; truncated files come
+ * back with a zero third row). */
+int mag_nersc_write(const char* path, const double* aos, mag_nersc_header* h);
+int mag_nersc_read(const char* path, double* aos, mag_nersc_header* h);
+/* export_config_nersc (field-matrix.h:452-568): plaquette and link trace of U
+ * computed on the device, links written from U. */
+int mag_export_config_nersc(mag_field U, const char* path, double beta, int sequence_number,
+                            const char* ensemble_label, int skip_third);
+/* import_config_nersc (field-matrix.h:592-713): dimensions must match U; the
+ * checksum must match; truncated links are rebuilt by reunitarize; the
+ * plaquette of U must agree with the header to 1e-4. h may be NULL. */
+int mag_import_config_nersc(mag_field U, const char* path, mag_nersc_header* h);
+
 /* -------------------------------------------------------------- serving
  * Continuous batching of independent chains (no counterpart in the
  * reference, whose statics allow one chain per process, field-hmc.h:911-922).
  * Chains are admitted in arrival order while their device footprint fits
- * the budget; every step runs one trajectory of each admitted chain, with
- * chains of one shape class batched into single launches (optionally
- * replayed from a HIP graph) and classes overlapping on several streams.
- * A chain's results are the same alone or in any mix. */
+ * the budget; chains of one shape class are batched into single launches
+ * (optionally replayed from a HIP graph), and every batch runs its
+ * trajectories on its own schedule on a pool of streams, so batches of
+ * small lattices do not wait for batches of large ones. A chain's results
+ * are the same alone or in any mix. */
 typedef struct {
   int dims[4];
   mag_action act;
@@ -205,7 +242,9 @@ int mag_server_create(mag_ctx ctx, size_t hbm_budget, int max_batch, int use_gra
 int mag_server_destroy(mag_server s);
 int mag_server_submit(mag_server s, const mag_serve_request* req, int* id);
 int mag_server_cancel(mag_server s, int id);
-/* admit, run one trajectory of every running chain, retire finished ones */
+/* admit pending chains, launch every batch not on the device, wait until at
+ * least one batch has finished its trajectory and process (accept, record,
+ * retire) every batch that has */
 int mag_server_step(mag_server s, int* active, int* pending);
 int mag_server_status(mag_server s, int id, int* state, int* done_traj);
 /* per-trajectory results and completion times (seconds since server creation) */

@@ -1,6 +1,7 @@
 // mag_amd.hpp — C++ host API with the reference's entry-point names
 // (hummingtree/mag field-hmc.h, field-matrix.h, field-wloop.h,
-// field-demon.h), as thin inline wrappers over the C ABI of mag_amd.h.
+// field-demon.h, NERSC I/O of field-matrix.h), as thin inline wrappers over
+// the C ABI of mag_amd.h.
 // A caller of the reference replaces qlat::Field<cps::Matrix> by
 // mag::Field and keeps its call sequence:
 //
@@ -171,6 +172,26 @@ inline std::pair<long, long> demon_microcanonical(Field& U, int nsweeps, double 
   long ar[2] = {0, 0};
   check(mag_demon_microcanonical(U.get(), nsweeps, E_max, seed, E.data(), ar), U.ctx());
   return {ar[0], ar[1]};
+}
+
+// Arg_export (field-matrix.h:445-450)
+struct Arg_export {
+  double beta = 0.;
+  int sequence_num = 0;
+  std::string ensemble_label;
+};
+// export_config_nersc (field-matrix.h:452)
+inline void export_config_nersc(const Field& U, const std::string& path, const Arg_export& arg,
+                                bool doesSkipThird = false) {
+  check(mag_export_config_nersc(U.get(), path.c_str(), arg.beta, arg.sequence_num, arg.ensemble_label.c_str(),
+                                doesSkipThird ? 1 : 0),
+        U.ctx());
+}
+// import_config_nersc (field-matrix.h:592)
+inline mag_nersc_header import_config_nersc(Field& U, const std::string& path) {
+  mag_nersc_header h{};
+  check(mag_import_config_nersc(U.get(), path.c_str(), &h), U.ctx());
+  return h;
 }
 
 // run_chmc (field-hmc.h:770) as a device-resident chain.

@@ -21,7 +21,8 @@ __all__ = [
     "get_force", "evolve_gauge_field", "evolve_momentum", "force_gradient_integrator", "leap_frog_integrator",
     "get_hamiltonian", "total_plaq", "total_rectangular", "avg_plaquette", "gauge_action", "kinetic_energy",
     "reunitarize", "init_momentum", "expiQ", "compute_Lambda", "stout_smear", "avg_wilson_loop",
-    "demon_microcanonical", "run_chmc",
+    "demon_microcanonical", "run_chmc", "Arg_export", "export_config_nersc", "import_config_nersc",
+    "nersc_write", "nersc_read",
 ]
 
 _D = C.POINTER(C.c_double)
@@ -357,6 +358,57 @@ class Chain:
         s = np.zeros(4)
         _check(_lib.lib().mag_chain_demon_state(self._h, _dptr(s)), self.ctx.handle)
         return s
+
+
+# ------------------------------------------------------------ NERSC I/O
+@dataclass
+class Arg_export:
+    """Arg_export (field-matrix.h:445-450)."""
+    beta: float = 0.0
+    sequence_num: int = 0
+    ensemble_label: str = ""
+
+
+def _header_dict(h):
+    return {"dims": tuple(h.dims), "skip_third": bool(h.skip_third), "link_trace": h.link_trace,
+            "plaquette": h.plaquette, "beta": h.beta, "sequence_number": h.sequence_number,
+            "ensemble_label": h.ensemble_label.decode(), "checksum": h.checksum, "crc32": h.crc32}
+
+
+def export_config_nersc(U, path, arg=None, does_skip_third=False):
+    """export_config_nersc (field-matrix.h:452-568)."""
+    arg = arg or Arg_export()
+    _check(_lib.lib().mag_export_config_nersc(U.handle, str(path).encode(), arg.beta, arg.sequence_num,
+                                              arg.ensemble_label.encode(), 1 if does_skip_third else 0),
+           U.ctx.handle)
+
+
+def import_config_nersc(U, path):
+    """import_config_nersc (field-matrix.h:592-713); returns the header as a dict."""
+    h = _lib.NerscHeader()
+    _check(_lib.lib().mag_import_config_nersc(U.handle, str(path).encode(), C.byref(h)), U.ctx.handle)
+    return _header_dict(h)
+
+
+def nersc_write(path, links, dims, skip_third=False, plaquette=0.0, link_trace=0.0, beta=0.0, sequence_number=0,
+                ensemble_label=""):
+    """Host-only NERSC writer for (V, 4, 18) links; returns (checksum, crc32)."""
+    a = np.ascontiguousarray(links, dtype=np.float64)
+    h = _lib.NerscHeader((C.c_int * 4)(*dims), 1 if skip_third else 0, link_trace, plaquette, beta,
+                         sequence_number, ensemble_label.encode())
+    _check(_lib.lib().mag_nersc_write(str(path).encode(), _dptr(a), C.byref(h)))
+    return h.checksum, h.crc32
+
+
+def nersc_read(path, header_only=False):
+    """Host-only NERSC reader: (links (V, 4, 18) or None, header dict)."""
+    h = _lib.NerscHeader()
+    _check(_lib.lib().mag_nersc_read(str(path).encode(), None, C.byref(h)))
+    if header_only:
+        return None, _header_dict(h)
+    a = np.empty((int(np.prod(h.dims)), 4, 18))
+    _check(_lib.lib().mag_nersc_read(str(path).encode(), _dptr(a), C.byref(h)))
+    return a, _header_dict(h)
 
 
 def run_chmc(ctx, U_host, arg, seed=1234, **kw):

@@ -11,9 +11,11 @@ LIB_PATH = os.path.join(_HERE, "libmag_amd.so")
 
 # status codes
 OK = 0
-ERR_ARG, ERR_GEOMETRY, ERR_OOM, ERR_DEVICE, ERR_NAN, ERR_UNSUPPORTED, ERR_BUDGET, ERR_STATE = range(-1, -9, -1)
+(ERR_ARG, ERR_GEOMETRY, ERR_OOM, ERR_DEVICE, ERR_NAN, ERR_UNSUPPORTED, ERR_BUDGET, ERR_STATE, ERR_IO,
+ ERR_CHECKSUM) = range(-1, -11, -1)
 STATUS_NAMES = {0: "MAG_OK", -1: "MAG_ERR_ARG", -2: "MAG_ERR_GEOMETRY", -3: "MAG_ERR_OOM", -4: "MAG_ERR_DEVICE",
-                -5: "MAG_ERR_NAN", -6: "MAG_ERR_UNSUPPORTED", -7: "MAG_ERR_BUDGET", -8: "MAG_ERR_STATE"}
+                -5: "MAG_ERR_NAN", -6: "MAG_ERR_UNSUPPORTED", -7: "MAG_ERR_BUDGET", -8: "MAG_ERR_STATE",
+                -9: "MAG_ERR_IO", -10: "MAG_ERR_CHECKSUM"}
 # field kinds / precisions / actions
 LINK18, LINK12, ALG8 = 0, 1, 2
 F32, F64 = 0, 1
@@ -35,6 +37,12 @@ class TrajResult(C.Structure):
     _fields_ = [("h_old", C.c_double), ("h_new", C.c_double), ("dh", C.c_double), ("die_roll", C.c_double),
                 ("accept", C.c_int), ("avg_plaq", C.c_double), ("s_gauge", C.c_double),
                 ("demon_e", C.c_double * 2), ("reunit_dev", C.c_double), ("status", C.c_int)]
+
+
+class NerscHeader(C.Structure):
+    _fields_ = [("dims", C.c_int * 4), ("skip_third", C.c_int), ("link_trace", C.c_double), ("plaquette", C.c_double),
+                ("beta", C.c_double), ("sequence_number", C.c_int), ("ensemble_label", C.c_char * 128),
+                ("checksum", C.c_uint32), ("crc32", C.c_uint32)]
 
 
 class ServeRequest(C.Structure):
@@ -92,6 +100,10 @@ SIGNATURES = {
     "mag_chain_demon_state": [_P, _D],
     "mag_chain_traj_index": [_P, _L],
     "mag_demon_microcanonical": [_P, C.c_int, C.c_double, C.c_uint64, _D, _L],
+    "mag_nersc_write": [C.c_char_p, _D, C.POINTER(NerscHeader)],
+    "mag_nersc_read": [C.c_char_p, _D, C.POINTER(NerscHeader)],
+    "mag_export_config_nersc": [_P, C.c_char_p, C.c_double, C.c_int, C.c_char_p, C.c_int],
+    "mag_import_config_nersc": [_P, C.c_char_p, C.POINTER(NerscHeader)],
     "mag_server_footprint": [C.POINTER(ServeRequest), C.POINTER(C.c_size_t)],
     "mag_server_create": [_P, C.c_size_t, C.c_int, C.c_int, C.POINTER(_P)],
     "mag_server_destroy": [_P],

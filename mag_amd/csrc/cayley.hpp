@@ -34,6 +34,7 @@ template <class T>
 struct CHC {
   bool zero;
   cpx<T> f0, f1, f2;
+  cpx<T> f0m1;  // f0 - 1, carried separately for exp(iQ) - 1
   cpx<T> b10, b11, b12, b20, b21, b22;
 };
 using CHCoef = CHC<double>;
@@ -47,7 +48,7 @@ MAG_HD CHCoef ch_coefficients(double c0, double c1) {
   if (c0 < 0.) { c0 = -c0; reflect = true; }
   c.zero = (c1 == 0.);
   if (c.zero) {
-    c.f0 = {1., 0.}; c.f1 = {0., 0.}; c.f2 = {0., 0.};
+    c.f0 = {1., 0.}; c.f1 = {0., 0.}; c.f2 = {0., 0.}; c.f0m1 = {0., 0.};
     c.b10 = c.b11 = c.b12 = c.b20 = c.b21 = c.b22 = {0., 0.};
     return c;
   }
@@ -114,6 +115,7 @@ MAG_HD CHCoef ch_coefficients(double c0, double c1) {
       c.b22 = {-c.b22.r, c.b22.i};
     }
   }
+  c.f0m1 = {c.f0.r - 1., c.f0.i};
   return c;
 }
 
@@ -135,7 +137,7 @@ template <bool NEED_B>
 MAG_HD CHC<float> ch_series(float c0, float c1) {
   CHC<float> c;
   c.zero = (c1 == 0.f);
-  float fr[3] = {1.f, 0.f, 0.f}, fi[3] = {0.f, 0.f, 0.f};
+  float fr[3] = {0.f, 0.f, 0.f}, fi[3] = {0.f, 0.f, 0.f};  // fr[0]: f0 - 1
   float pr[3] = {0.f, 0.f, 0.f}, pi[3] = {0.f, 0.f, 0.f};
   float qr[3] = {0.f, 0.f, 0.f}, qi[3] = {0.f, 0.f, 0.f};
   float a = 1.f, b = 0.f, d = 0.f;                  // (iQ)^n / n! in units of i^n
@@ -157,11 +159,12 @@ MAG_HD CHC<float> ch_series(float c0, float c1) {
     add_phase(n, a, fr[0], fi[0]); add_phase(n, b, fr[1], fi[1]); add_phase(n, d, fr[2], fi[2]);
   }
   if (c.zero) {
-    c.f0 = {1.f, 0.f}; c.f1 = {0.f, 0.f}; c.f2 = {0.f, 0.f};
+    c.f0 = {1.f, 0.f}; c.f1 = {0.f, 0.f}; c.f2 = {0.f, 0.f}; c.f0m1 = {0.f, 0.f};
     c.b10 = c.b11 = c.b12 = c.b20 = c.b21 = c.b22 = {0.f, 0.f};
     return c;
   }
-  c.f0 = {fr[0], fi[0]}; c.f1 = {fr[1], fi[1]}; c.f2 = {fr[2], fi[2]};
+  c.f0m1 = {fr[0], fi[0]};
+  c.f0 = {1.f + fr[0], fi[0]}; c.f1 = {fr[1], fi[1]}; c.f2 = {fr[2], fi[2]};
   c.b10 = {pr[0], pi[0]}; c.b11 = {pr[1], pi[1]}; c.b12 = {pr[2], pi[2]};
   c.b20 = {qr[0], qi[0]}; c.b21 = {qr[1], qi[1]}; c.b22 = {qr[2], qi[2]};
   return c;
@@ -172,6 +175,7 @@ MAG_HD CHC<T> to_coef(const CHC<double>& c) {
   CHC<T> r;
   r.zero = c.zero;
   r.f0 = {T(c.f0.r), T(c.f0.i)}; r.f1 = {T(c.f1.r), T(c.f1.i)}; r.f2 = {T(c.f2.r), T(c.f2.i)};
+  r.f0m1 = {T(c.f0m1.r), T(c.f0m1.i)};
   r.b10 = {T(c.b10.r), T(c.b10.i)}; r.b11 = {T(c.b11.r), T(c.b11.i)}; r.b12 = {T(c.b12.r), T(c.b12.i)};
   r.b20 = {T(c.b20.r), T(c.b20.i)}; r.b21 = {T(c.b21.r), T(c.b21.i)}; r.b22 = {T(c.b22.r), T(c.b22.i)};
   return r;
@@ -224,6 +228,49 @@ MAG_HD M3<T> expiQ(const M3<T>& Q) {
   const CHC<T> c = ch_coef<false>(c0, c1);
   if (c.zero) return unit3<T>();
   return ch_combine(Q, Q2, c.f0, c.f1, c.f2);
+}
+
+// exp(iQ) - 1 = (f0 - 1) + f1 Q + f2 Q^2, never forming 1 + (small).
+template <class T>
+MAG_HD M3<T> expiQ_m1(const M3<T>& Q) {
+  const M3<T> Q2 = mul(Q, Q);
+  T c0, c1;
+  ch_invariants(Q, Q2, c0, c1);
+  const CHC<T> c = ch_coef<false>(c0, c1);
+  M3<T> r;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r.e[k] = cfma(c.f2, Q2.e[k], c.f1 * Q.e[k]);
+  if (c.zero) return zero3<T>();
+#pragma unroll
+  for (int i = 0; i < 3; ++i) r(i, i) = r(i, i) + c.f0m1;
+  return r;
+}
+
+// Link updates exp(iQ) U and U exp(iQ). FP64 multiplies by exp(iQ), as the
+// reference does. FP32 adds (exp(iQ) - 1) U to U: forming exp(iQ) first
+// rounds the O(|Q|^2) parts of f0 = 1 - c1/2 + ... against 1, and over a
+// trajectory that shrinks every link by ~2e-7 in the same direction (a
+// systematic shift of the FP32 dH); U + (exp(iQ) - 1) U rounds once per
+// element with no bias.
+template <class T>
+MAG_HD M3<T> exp_left(const M3<T>& Q, const M3<T>& U) {
+  if constexpr (sizeof(T) == sizeof(float)) {
+    M3<T> r = U;
+    mul_acc(r, expiQ_m1(Q), U);
+    return r;
+  } else {
+    return mul(expiQ(Q), U);
+  }
+}
+template <class T>
+MAG_HD M3<T> exp_right(const M3<T>& U, const M3<T>& Q) {
+  if constexpr (sizeof(T) == sizeof(float)) {
+    M3<T> r = U;
+    mul_acc(r, U, expiQ_m1(Q));
+    return r;
+  } else {
+    return mul(U, expiQ(Q));
+  }
 }
 
 // Lambda of compute_Lambda (field-multi.h:180-269), returned as its 8

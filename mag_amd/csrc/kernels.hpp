@@ -25,6 +25,15 @@ namespace mag {
 
 constexpr int BLOCK = 256;
 
+// The FP32 stencil kernels need ~150-170 VGPRs. The waves per SIMD follow
+// from the VGPR budget (512 per lane): <= 168 gives 3 waves, above that 2.
+// FP32 instances are held to 3 waves so that none falls to 2 (which cost
+// the Iwasaki top kernel 20 %); FP64 instances (compute type double) keep
+// their registers, since holding them to 168 spills hundreds of bytes.
+template <class TC>
+constexpr int stencil_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
+#define MAG_STENCIL_OCC __attribute__((amdgpu_waves_per_eu(stencil_waves<TC>())))
+
 // Blocks are dispatched round-robin over the 8 XCDs (XCD = linear block id
 // mod 8); XCD k gets tiles [k nb/8, (k+1) nb/8) in order.
 MAG_D void stencil_tile(int& mu, int& sblk) {
@@ -165,7 +174,7 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 #pragma unroll
     for (int k = 0; k < 8; ++k) f[k] *= e;
     const M3<TC> Q = alg_to_mat(f);
-    a.Uaux.store(mu, s, mul(expiQ(Q), Ux));
+    a.Uaux.store(mu, s, exp_left(Q, Ux));
   }
 }
 
@@ -205,9 +214,9 @@ MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   force_epilogue<TS, NC, TC>(a, mu, s, Ux, f);
 }
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, TS, NC, TC>(a); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, TS, NC, TC>(a); }
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
 
 // ============================================================ link updates
 // evolve_gauge_field (field-hmc.h:442-482) with the constraint classes:
@@ -238,14 +247,14 @@ MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
     const TC sg = (cls == 0) ? e : -e;
 #pragma unroll
     for (int k = 0; k < 8; ++k) h[k] *= sg;
-    const M3<TC> E = expiQ(alg_to_mat(h));
-    U = (cls == 0) ? mul(E, U) : mul(U, E);
+    const M3<TC> Qh = alg_to_mat(h);
+    U = (cls == 0) ? exp_left(Qh, U) : exp_right(U, Qh);
   }
   if (cls == 1 || cls == 100) {
     a.P.load(mu, s + off_rt(a.g, p, mu, -1), h);
 #pragma unroll
     for (int k = 0; k < 8; ++k) h[k] *= e;
-    U = mul(expiQ(alg_to_mat(h)), U);
+    U = exp_left(alg_to_mat(h), U);
   }
   a.Uout.store(mu, s, U);
 }
@@ -308,16 +317,16 @@ MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) q[k] *= rho;
   const M3<TC> Q = alg_to_mat(q);
-  a.Unext.store(mu, s, mul(expiQ(Q), Ux));
+  a.Unext.store(mu, s, exp_left(Q, Ux));
   if (a.store_qc) {
     a.Q.store(mu, s, q);
     a.C.store(mu, s, scale(S, rho));
   }
 }
 template <class TSI, int NCI, class TSO, int NCO, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(a); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(a); }
 template <class TSI, int NCI, class TSO, int NCO, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_fwd_b(const StoutFwdArgs<TSI, NCI, TSO, NCO>* t) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(t[blockIdx.z]); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd_b(const StoutFwdArgs<TSI, NCI, TSO, NCO>* t) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(t[blockIdx.z]); }
 
 // Lambda step of level k for one link, given Z' = dS/dU_{k+1} there
 // (compute_Lambda, field-multi.h:180-269, eqs. 72-73):
@@ -370,9 +379,9 @@ MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
   stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
 }
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_top(StoutTopArgs<TS, NC> a) { k_stout_top_body<ACT, TS, NC, TC>(a); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top(StoutTopArgs<TS, NC> a) { k_stout_top_body<ACT, TS, NC, TC>(a); }
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_top_b(const StoutTopArgs<TS, NC>* t) { k_stout_top_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top_b(const StoutTopArgs<TS, NC>* t) { k_stout_top_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
 
 // Backward level k (eq. 75): Z_k = W_k + sum over nu of the six
 // Lambda-inserted staple terms. With Y = -i rho U^+ Lambda every Y sits next
@@ -490,9 +499,9 @@ MAG_D void k_stout_bwd_body(const StoutBwdArgs<TS, NC>& a) {
   }
 }
 template <bool LAST, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_bwd(StoutBwdArgs<TS, NC> a) { k_stout_bwd_body<LAST, TS, NC, TC>(a); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_bwd(StoutBwdArgs<TS, NC> a) { k_stout_bwd_body<LAST, TS, NC, TC>(a); }
 template <bool LAST, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_stout_bwd_b(const StoutBwdArgs<TS, NC>* t) { k_stout_bwd_body<LAST, TS, NC, TC>(t[blockIdx.z]); }
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_bwd_b(const StoutBwdArgs<TS, NC>* t) { k_stout_bwd_body<LAST, TS, NC, TC>(t[blockIdx.z]); }
 
 // Constrained runs with smearing: F from a stored Z_0 with the classes of
 // field-hmc.h:353-378: W = U Z (0), U(x+mu) Z(x+mu) - Z U (1, 10), 0 (100);

@@ -10,6 +10,8 @@
 //   algebra fields (momenta, forces, stout Q, Lambda)  NC = 8, H = sum h_a T_a
 //   general matrix fields (stout C, Y, W, Z)          NC = 18
 #pragma once
+#include <type_traits>
+
 #include "su3.hpp"
 
 #define MAG_D __device__ __forceinline__
@@ -87,29 +89,45 @@ MAG_D gp<T> gptr(T* p) { return (gp<T>)p; }
 template <class T>
 MAG_D gp<const T> gptr(const T* p) { return (gp<const T>)p; }
 
+// Component c of direction mu at site s. The plane pointer p + (mu NC + c) V
+// is wavefront-uniform whenever mu is, and the site enters as a 32-bit
+// unsigned offset, so the load/store takes the scalar-base + vector-offset
+// form with one offset register shared by all components of a site (no
+// per-component 64-bit address arithmetic on the vector ALUs).
+template <class T>
+MAG_D gp<T> plane(T* p, size_t plane_index, int V) { return gptr(p + plane_index * (size_t)V); }
+// Element at a 32-bit byte offset from a plane (planes hold < 2^30 reals).
+template <class T>
+using gref = __attribute__((address_space(1))) T&;
+template <class T>
+MAG_D gref<T> at(gp<T> base, unsigned s) {
+  using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return *(gp<T>)((gp<C>)base + s * (unsigned)sizeof(T));
+}
+
 template <class TS, int NC>
 struct LinkView {
   TS* p;
   int V;
   template <class TC>
   MAG_D M3<TC> load(int mu, int s) const {
-    const auto b = gptr(p + (size_t)mu * NC * V + s);
+    const unsigned u = (unsigned)s;
     M3<TC> m;
 #pragma unroll
     for (int k = 0; k < NC / 2; ++k) {
-      m.e[k].r = TC(b[(size_t)(2 * k) * V]);
-      m.e[k].i = TC(b[(size_t)(2 * k + 1) * V]);
+      m.e[k].r = TC(at(plane(p, (size_t)(mu * NC + 2 * k), V), u));
+      m.e[k].i = TC(at(plane(p, (size_t)(mu * NC + 2 * k + 1), V), u));
     }
     if (NC == 12) reconstruct_row2(m);
     return m;
   }
   template <class TC>
   MAG_D void store(int mu, int s, const M3<TC>& m) const {
-    const auto b = gptr(p + (size_t)mu * NC * V + s);
+    const unsigned u = (unsigned)s;
 #pragma unroll
     for (int k = 0; k < NC / 2; ++k) {
-      b[(size_t)(2 * k) * V] = TS(m.e[k].r);
-      b[(size_t)(2 * k + 1) * V] = TS(m.e[k].i);
+      at(plane(p, (size_t)(mu * NC + 2 * k), V), u) = TS(m.e[k].r);
+      at(plane(p, (size_t)(mu * NC + 2 * k + 1), V), u) = TS(m.e[k].i);
     }
   }
 };
@@ -120,15 +138,15 @@ struct AlgView {
   int V;
   template <class TC>
   MAG_D void load(int mu, int s, TC h[8]) const {
-    const auto b = gptr(p + (size_t)mu * 8 * V + s);
+    const unsigned u = (unsigned)s;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) h[a] = TC(b[(size_t)a * V]);
+    for (int a = 0; a < 8; ++a) h[a] = TC(at(plane(p, (size_t)(mu * 8 + a), V), u));
   }
   template <class TC>
   MAG_D void store(int mu, int s, const TC h[8]) const {
-    const auto b = gptr(p + (size_t)mu * 8 * V + s);
+    const unsigned u = (unsigned)s;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) b[(size_t)a * V] = TS(h[a]);
+    for (int a = 0; a < 8; ++a) at(plane(p, (size_t)(mu * 8 + a), V), u) = TS(h[a]);
   }
   template <class TC>
   MAG_D M3<TC> load_mat(int mu, int s) const {

@@ -11,10 +11,15 @@
 //     and emitted as one launch per kernel over all its chains
 //     (blockIdx.z = chain), optionally captured in a HIP graph that is
 //     replayed while the membership is unchanged;
-//   - overlap: every batch is in flight on its own schedule. It goes to the
-//     least-loaded of a pool of streams, and its completion is an event, so
-//     a batch of small lattices runs its next trajectory as soon as its last
-//     one finished, without waiting for batches of large lattices;
+//   - overlap: every batch is in flight on its own schedule and its
+//     completion is an event, so a batch of small lattices runs its next
+//     trajectory as soon as its last one finished. Batches of large lattices
+//     (whose kernels fill the device by themselves) queue on one stream;
+//     small batches go to the less loaded of two others, so they never wait
+//     in a hardware queue behind a large trajectory. Admission, table
+//     uploads and graph capture run on the context's own stream, and every
+//     host->device copy of a launch comes from pinned memory, so launching
+//     never blocks the host;
 //   - completion: a finished batch's FP64 scalars (one copy per batch) feed
 //     the accept decisions on the host; rejected chains are restored on the
 //     batch's stream; finished, cancelled and non-finite chains retire, and
@@ -40,7 +45,8 @@ using namespace mag;
 
 namespace {
 
-constexpr int NSTREAMS = 4;
+constexpr int NSTREAMS = 3;             // stream 0: large batches; 1, 2: small ones
+constexpr double LARGE_SITES = 65536.;  // a batch of >= 16^4 sites in all fills the device
 
 enum { ST_PENDING = 0, ST_RUNNING = 1, ST_DONE = 2, ST_FAILED = 3, ST_CANCELLED = 4 };
 
@@ -83,6 +89,7 @@ struct Batch {
   double* gather = nullptr;  // S_COUNT doubles per member (device)
   size_t gather_n = 0;
   double* h_gather = nullptr;
+  uint64_t* h_epoch = nullptr;  // pinned: trajectory index of each member
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t done = nullptr;
@@ -104,7 +111,7 @@ struct mag_server_s {
   int max_batch = 64;
   bool graphs = true;
   hipStream_t streams[NSTREAMS] = {};
-  hipStream_t aux = nullptr;  // admission, table uploads, graph capture
+  hipStream_t aux = nullptr;  // the context's stream: admission, table uploads, graph capture
   double load[NSTREAMS] = {};
   std::vector<std::unique_ptr<Entry>> entries;
   std::deque<Entry*> pending;
@@ -128,10 +135,12 @@ void release_tables(mag_server s, Batch* b) {
   if (b->tables) dev_free(s->ctx, b->tables, b->table_bytes);
   if (b->gather) dev_free(s->ctx, b->gather, b->gather_n * S_COUNT * sizeof(double));
   if (b->h_gather) (void)hipHostFree(b->h_gather);
+  if (b->h_epoch) (void)hipHostFree(b->h_epoch);
   b->tables = nullptr;
   b->gather = nullptr;
   b->gather_n = 0;
   b->h_gather = nullptr;
+  b->h_epoch = nullptr;
   b->rec = Recorder();
   b->valid = false;
 }
@@ -154,7 +163,8 @@ int build_batch(mag_server s, Batch* b) {
     return fail(x, MAG_ERR_DEVICE, "event creation failed");
   if ((r = dev_alloc(x, (void**)&b->gather, n * S_COUNT * sizeof(double)))) return r;
   b->gather_n = n;
-  if (hipHostMalloc(&b->h_gather, n * S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc(&b->h_gather, n * S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&b->h_epoch, n * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
     return fail(x, MAG_ERR_OOM, "pinned gather buffer");
   b->rec = Recorder();
   for (size_t i = 0; i < n; ++i) {
@@ -198,17 +208,20 @@ int build_batch(mag_server s, Batch* b) {
 int launch_batch(mag_server s, Batch* b) {
   mag_ctx x = s->ctx;
   const size_t n = b->members.size();
+  double sites = 0.;
+  for (Entry* e : b->members)
+    sites += (double)e->req.dims[0] * e->req.dims[1] * e->req.dims[2] * e->req.dims[3];
   int best = 0;
-  for (int i = 1; i < NSTREAMS; ++i)
-    if (s->load[i] < s->load[best]) best = i;
+  if (sites < LARGE_SITES) best = (s->load[1] <= s->load[2]) ? 1 : 2;
   b->sidx = best;
   hipStream_t st = s->streams[best];
   for (Entry* e : b->members) e->chain->ctx->stream = st;
-  // trajectory index of every member (read by the momentum kernel)
+  // trajectory index of every member (read by the momentum kernel), from
+  // pinned memory: the copies are asynchronous
   for (size_t i = 0; i < n; ++i) {
     mag_chain c = b->members[i]->chain;
-    uint64_t e = (uint64_t)c->traj;
-    HIPCHK(x, hipMemcpyAsync(c->d_epoch, &e, sizeof e, hipMemcpyHostToDevice, st));
+    b->h_epoch[i] = (uint64_t)c->traj;
+    HIPCHK(x, hipMemcpyAsync(c->d_epoch, b->h_epoch + i, sizeof(uint64_t), hipMemcpyHostToDevice, st));
   }
   if (b->exec) {
     HIPCHK(x, hipGraphLaunch(b->exec, st));
@@ -439,10 +452,7 @@ int mag_server_create(mag_ctx ctx, size_t hbm_budget, int max_batch, int use_gra
       mag_server_destroy(s);
       return fail(ctx, MAG_ERR_DEVICE, "stream creation failed");
     }
-  if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess) {
-    mag_server_destroy(s);
-    return fail(ctx, MAG_ERR_DEVICE, "stream creation failed");
-  }
+  s->aux = ctx->stream;
   *out = s;
   return MAG_OK;
 }
@@ -460,7 +470,6 @@ int mag_server_destroy(mag_server s) {
   (void)reap(s, true);
   for (auto& st : s->streams)
     if (st) (void)hipStreamDestroy(st);
-  if (s->aux) (void)hipStreamDestroy(s->aux);
   delete s;
   return MAG_OK;
 }

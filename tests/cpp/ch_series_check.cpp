@@ -42,6 +42,19 @@ int main() {
     const double re = b.f0.r + b.f1.r * x + b.f2.r * x * x, im = b.f0.i + b.f1.i * x + b.f2.i * x * x;
     ed = std::fmax(ed, std::hypot(re - std::cos(x), im - std::sin(x)));
   }
-  std::printf("f %.3e  b1 %.3e  b2 %.3e  diag %.3e\n", ef, e1, e2, ed);
-  return (ef < 2e-7 && e1 < 2e-7 && e2 < 2e-7 && ed < 2e-7) ? 0 : 1;
+  // exp(iQ) - 1 at small |Q| (the FP32 link update U + (exp(iQ) - 1) U):
+  // relative to |e^{iq} - 1| on a diagonal Q, e^{iq} - 1 = (-2 sin^2(q/2), sin q).
+  double em = 0.;
+  for (double scale : {1e-5, 1e-3, 1e-1}) {
+    const double d[3] = {0.7 * scale, -0.2 * scale, -0.5 * scale};
+    const double k1 = (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / 2., k0 = d[0] * d[1] * d[2];
+    const CHC<float> m = ch_series<false>(float(k0), float(k1));
+    for (double x : d) {
+      const double re = m.f0m1.r + m.f1.r * x + m.f2.r * x * x, im = m.f0m1.i + m.f1.i * x + m.f2.i * x * x;
+      const double sh = std::sin(x / 2.);
+      em = std::fmax(em, std::hypot(re + 2. * sh * sh, im - std::sin(x)) / std::fabs(x));
+    }
+  }
+  std::printf("f %.3e  b1 %.3e  b2 %.3e  diag %.3e  expm1 %.3e\n", ef, e1, e2, ed, em);
+  return (ef < 2e-7 && e1 < 2e-7 && e2 < 2e-7 && ed < 2e-7 && em < 1e-6) ? 0 : 1;
 }

@@ -333,3 +333,81 @@ def test_gpu_against_golden_fixtures(ctx, orc):
     res = ch.run(20)
     assert [r.accept for r in res] == list(s["accept"])
     assert np.allclose([r.dh for r in res], s["dh"], rtol=1e-6, atol=1e-9)
+
+
+def test_c2_fp32_md_against_fp64_16x4(ctx, orc):
+    """16^4 Iwasaki (c1 = -0.331, beta = 2.6) + 3-stout (rho = 0.1): Haar seed 7,
+    20 FP64 thermalisation trajectories, then 50 trajectories (tau = 1, FG,
+    10 steps) twice from the same start and seed, all-FP64 and FP32-MD /
+    FP64-dH. Force and plaquette of the thermalised links against the FP64
+    oracle; accept decisions matched within 2 %; dH agreement; <exp(-dH)> = 1."""
+    L16 = (16, 16, 16, 16)
+    kw = dict(trajectory_length=10, dt=0.1, beta=2.6, gauge=M.Gauge(M.IWASAKI, -0.331), n_stout=3, rho=0.1)
+    therm = M.Chain(ctx, L16, M.Arg_chmc(num_forced_accept_step=20, **kw), seed=7, precision=F64)
+    therm.haar(7)
+    therm.run(20)
+    U = therm.get_links()
+    a = orc.action(type=1, beta=2.6, c1=-0.331, n_stout=3, rho=0.1)
+    act = M._lib.Action(1, 2.6, -0.331, 0, 3, 0.1)
+    ref = orc.get_force(L16, U, a)
+    F = alg(ctx, L16)
+    M.get_force(F, links(ctx, L16, U), act)
+    assert rel(F.download(), ref) < 1e-12
+    U32 = links(ctx, L16, U, F32)
+    F32f = alg(ctx, L16, prec=F32)
+    M.get_force(F32f, U32, act)
+    assert rel(F32f.download(), orc.get_force(L16, U32.download(), a)) < 2e-5
+    assert rel(M.avg_plaquette(U32), orc.avg_plaquette(L16, U)) < 1e-7
+
+    arg = M.Arg_chmc(**kw)
+    res = {}
+    for prec in (F64, F32):
+        ch = M.Chain(ctx, L16, arg, seed=8, precision=prec)
+        ch.set_links(U)
+        res[prec] = ch.run(50)
+    a64 = np.array([r.accept for r in res[F64]])
+    a32 = np.array([r.accept for r in res[F32]])
+    dh64 = np.array([r.dh for r in res[F64]])
+    dh32 = np.array([r.dh for r in res[F32]])
+    info = f"accept64={a64.astype(int).tolist()} accept32={a32.astype(int).tolist()} dh64={dh64.round(4).tolist()} " \
+           f"dh32={dh32.round(4).tolist()}"
+    assert a64.mean() > 0.2, info
+    # The first trajectories start from the same links (to FP32 rounding) and
+    # integrate the same momenta: there the FP32-MD dH and the Metropolis
+    # probability min(1, exp(-dH)) must match the FP64 ones. Later the two
+    # chains drift apart (molecular dynamics amplifies the rounding
+    # differences from one trajectory to the next) and become independent
+    # samples whose acceptance rates agree statistically.
+    n = int(np.argmax(a64 != a32)) if np.any(a64 != a32) else len(a64)
+    assert n >= 10, info
+    n = 10
+    assert np.abs(dh32[:n] - dh64[:n]).max() < 0.02, info
+    p64, p32 = np.minimum(1, np.exp(-dh64[:n])), np.minimum(1, np.exp(-dh32[:n]))
+    assert np.abs(p32 - p64).max() < 0.02, info
+    assert abs(a32.mean() - a64.mean()) <= 0.1, info
+    w = np.exp(-dh64)
+    assert abs(w.mean() - 1.0) < 4 * w.std(ddof=1) / np.sqrt(len(w)) + 1e-3, info
+
+
+def test_c5_shape_64x4_dbw2_3stout_fp32(ctx):
+    """64^4 DBW2 (c1 = -1.4069, beta = 1.04) + 3-stout FP32 chain from a Haar
+    start (seed 99): one forced-accept force-gradient trajectory with 16 steps
+    at the full volume (16.7 M sites, ~78 GB resident), then the 4x4 Wilson
+    loop. Checks that every index and byte offset of the large lattice stays
+    in range: finite energies, unitary links, and a Wilson loop that matches
+    the FP64 re-evaluation of the same (FP32) links."""
+    L64 = (64, 64, 64, 64)
+    arg = M.Arg_chmc(trajectory_length=16, dt=1 / 16, beta=1.04, gauge=M.Gauge(M.IWASAKI, -1.4069), n_stout=3,
+                     rho=0.1, num_forced_accept_step=1)
+    ch = M.Chain(ctx, L64, arg, seed=99, precision=F32)
+    ch.haar(99)
+    r = ch.run(1)[0]
+    assert np.isfinite(r.h_old) and np.isfinite(r.h_new) and r.accept == 1
+    assert abs(r.dh) < 1e-3 * abs(r.h_old)
+    assert r.reunit_dev < 1e-4
+    U = ch.links()
+    w32 = M.avg_wilson_loop(U, (4, 0, 0), 4)
+    U64 = M.Field(ctx, L64, LINK18, F64).copy_from(U)
+    w64 = M.avg_wilson_loop(U64, (4, 0, 0), 4)
+    assert np.isfinite(w32) and abs(w32 - w64) < 1e-12
+    assert abs(M.avg_plaquette(U64) - r.avg_plaq) < 1e-12
