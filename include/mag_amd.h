@@ -155,6 +155,14 @@ int mag_kinetic_energy(mag_field P, int mag, double* K);
 int mag_hamiltonian(mag_field U, mag_field P, const mag_action* act, double parts[2], double* H);
 /* avg_wilson_loop (field-wloop.h:58-98) */
 int mag_avg_wilson_loop(mag_field U, const int r[3], int T, double* out);
+/* Asynchronous variants: the result stays on the device, written to d_out
+ * (device memory) in the context's stream order; nothing synchronises.
+ * total_plaq / total_rectangular / avg_wilson_loop: d_out[0];
+ * hamiltonian: d_out[0..2] = {kinetic, potential, H}. */
+int mag_total_plaq_async(mag_field U, double* d_out);
+int mag_total_rectangular_async(mag_field U, double* d_out);
+int mag_hamiltonian_async(mag_field U, mag_field P, const mag_action* act, double* d_out);
+int mag_avg_wilson_loop_async(mag_field U, const int r[3], int T, double* d_out);
 
 /* -------------------------------------------------------------- HMC chain
  * run_chmc / update_field (field-hmc.h:770-986) as a device-resident chain:
@@ -182,6 +190,22 @@ int mag_chain_traj_index(mag_chain c, long* next_traj);
  * device; E[4] in/out (plaquette, rectangle, chair, twist). */
 int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
                              long acc_rej[2]);
+
+/* ------------------------------------------------------ mag-blocked lattices
+ * Coarse/fine tools of the constrained HMC (link classes of is_constrained,
+ * field-hmc.h:309-337). The fine field's extents are mag times the coarse
+ * field's; both fields share format and context. */
+/* coarse link = ordered product of the mag fine links from mag*xC
+ * (coarse initialisation of double_multigrid, field-hmc.h:1068-1078) */
+int mag_coarsen_links(mag_field fine, int mag, mag_field coarse);
+/* update_constrain (field-hmc.h:988-1009): first fine link of each coarse
+ * link := U_c [U(x+mu) ... U(x+(mag-1)mu)]^+ */
+int mag_update_constrain(mag_field fine, mag_field coarse, int mag);
+/* check_constrained_plaquette (field-matrix.h:407-443) */
+int mag_constrained_plaquette(mag_field U, int mag, double* out);
+/* lattice_expansion (expansion/main.C:19-62) without the file I/O: unit
+ * links, the coarse link at fine site mag*x */
+int mag_lattice_expansion(mag_field coarse, int mag, mag_field fine);
 
 /* ---------------------------------------------------------- NERSC config I/O
  * export_config_nersc / import_config_nersc (field-matrix.h:445-713): text

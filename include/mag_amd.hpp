@@ -134,6 +134,11 @@ inline double get_hamiltonian(const Field& U, const Field& P, const Arg_chmc& ar
   part.assign(parts, parts + 2);
   return H;
 }
+// get_hamiltonian left on the device: d_out[0..2] = {kinetic, potential, H}
+inline void get_hamiltonian_async(const Field& U, const Field& P, const Arg_chmc& arg, double* d_out) {
+  mag_action a = arg.action();
+  check(mag_hamiltonian_async(U.get(), P.get(), &a, d_out), U.ctx());
+}
 // init_momentum (field-hmc.h:646), Philox stream keyed by (seed, trajectory)
 inline void init_momentum(Field& P, uint64_t seed, uint64_t traj, double m = 1.) {
   check(mag_init_momentum(P.get(), seed, traj, m), P.ctx());
@@ -154,6 +159,15 @@ inline double avg_plaquette(const Field& U) {
   check(mag_avg_plaquette(U.get(), &v), U.ctx());
   return v;
 }
+// Asynchronous variants: results stay in device memory (d_out), written in
+// the context's stream order.
+inline void total_plaq_async(const Field& U, double* d_out) { check(mag_total_plaq_async(U.get(), d_out), U.ctx()); }
+inline void total_rectangular_async(const Field& U, double* d_out) {
+  check(mag_total_rectangular_async(U.get(), d_out), U.ctx());
+}
+inline void avg_wilson_loop_async(const Field& U, const std::array<int, 3>& r, int T, double* d_out) {
+  check(mag_avg_wilson_loop_async(U.get(), r.data(), T, d_out), U.ctx());
+}
 // reunitarize (field-matrix.h:83)
 inline double reunitarize(Field& U) {
   double v = 0.;
@@ -172,6 +186,23 @@ inline std::pair<long, long> demon_microcanonical(Field& U, int nsweeps, double 
   long ar[2] = {0, 0};
   check(mag_demon_microcanonical(U.get(), nsweeps, E_max, seed, E.data(), ar), U.ctx());
   return {ar[0], ar[1]};
+}
+
+// ------------------------------------------------- mag-blocked lattices
+// coarse links as products of mag fine links (double_multigrid's coarse
+// initialisation, field-hmc.h:1068-1078)
+inline void coarsen_links(const Field& fine, int mag, Field& coarse) {
+  check(mag_coarsen_links(fine.get(), mag, coarse.get()), fine.ctx());
+}
+// update_constrain (field-hmc.h:988)
+inline void update_constrain(Field& gField, const Field& gField_coarse, const Arg_chmc& arg) {
+  check(mag_update_constrain(gField.get(), gField_coarse.get(), arg.mag), gField.ctx());
+}
+// check_constrained_plaquette (field-matrix.h:407)
+inline double check_constrained_plaquette(const Field& gField, int mag) {
+  double v = 0.;
+  check(mag_constrained_plaquette(gField.get(), mag, &v), gField.ctx());
+  return v;
 }
 
 // Arg_export (field-matrix.h:445-450)

@@ -22,7 +22,9 @@ __all__ = [
     "get_hamiltonian", "total_plaq", "total_rectangular", "avg_plaquette", "gauge_action", "kinetic_energy",
     "reunitarize", "init_momentum", "expiQ", "compute_Lambda", "stout_smear", "avg_wilson_loop",
     "demon_microcanonical", "run_chmc", "Arg_export", "export_config_nersc", "import_config_nersc",
-    "nersc_write", "nersc_read",
+    "nersc_write", "nersc_read", "total_plaq_async", "total_rectangular_async", "get_hamiltonian_async",
+    "avg_wilson_loop_async", "coarsen_links", "update_constrain", "check_constrained_plaquette",
+    "lattice_expansion",
 ]
 
 _D = C.POINTER(C.c_double)
@@ -270,6 +272,27 @@ def avg_wilson_loop(U, r, T):
     return _scalar(_lib.lib().mag_avg_wilson_loop, U.handle, (C.c_int * 3)(*r), T, ctx=U.ctx.handle)
 
 
+# Asynchronous reductions: d_out is a device address (an integer, e.g. a
+# torch tensor's data_ptr()); the result is written in the context's stream
+# order and nothing synchronises.
+def total_plaq_async(U, d_out):
+    _check(_lib.lib().mag_total_plaq_async(U.handle, C.c_void_p(d_out)), U.ctx.handle)
+
+
+def total_rectangular_async(U, d_out):
+    _check(_lib.lib().mag_total_rectangular_async(U.handle, C.c_void_p(d_out)), U.ctx.handle)
+
+
+def get_hamiltonian_async(U, P, arg, d_out):
+    """d_out[0:3] = (kinetic, potential, H)."""
+    a = _act(arg)
+    _check(_lib.lib().mag_hamiltonian_async(U.handle, P.handle, C.byref(a), C.c_void_p(d_out)), U.ctx.handle)
+
+
+def avg_wilson_loop_async(U, r, T, d_out):
+    _check(_lib.lib().mag_avg_wilson_loop_async(U.handle, (C.c_int * 3)(*r), T, C.c_void_p(d_out)), U.ctx.handle)
+
+
 def _batch(ctx, fn, mats, prec):
     arrs = [np.ascontiguousarray(np.asarray(m, dtype=np.complex128)).view(np.float64).reshape(-1, 18) for m in mats]
     n = arrs[0].shape[0]
@@ -358,6 +381,43 @@ class Chain:
         s = np.zeros(4)
         _check(_lib.lib().mag_chain_demon_state(self._h, _dptr(s)), self.ctx.handle)
         return s
+
+
+# --------------------------------------------------- mag-blocked lattices
+def _mag(arg_or_mag):
+    return arg_or_mag if isinstance(arg_or_mag, int) else arg_or_mag.mag
+
+
+def coarsen_links(fine, mag, coarse):
+    """Coarse links = ordered products of mag fine links (double_multigrid's
+    coarse initialisation, field-hmc.h:1068-1078)."""
+    _check(_lib.lib().mag_coarsen_links(fine.handle, mag, coarse.handle), fine.ctx.handle)
+    return coarse
+
+
+def update_constrain(gField, gField_coarse, arg):
+    """update_constrain (field-hmc.h:988-1009); arg is an Arg_chmc (its mag) or mag."""
+    _check(_lib.lib().mag_update_constrain(gField.handle, gField_coarse.handle, _mag(arg)), gField.ctx.handle)
+
+
+def check_constrained_plaquette(gField, mag):
+    """check_constrained_plaquette (field-matrix.h:407-443)."""
+    return _scalar(_lib.lib().mag_constrained_plaquette, gField.handle, mag, ctx=gField.ctx.handle)
+
+
+def lattice_expansion(mag, original, expanded, original_config=None, expanded_config=None):
+    """lattice_expansion (expansion/main.C:19-62): expanded = unit links with
+    the original links at the fine sites mag*x. With file names, the original
+    is imported from and the expansion exported to NERSC files (two-row
+    format, label "lattice expansion"), as the reference does. Returns
+    (average plaquette, constrained plaquette) of the expansion."""
+    if original_config is not None:
+        import_config_nersc(original, original_config)
+    _check(_lib.lib().mag_lattice_expansion(original.handle, mag, expanded.handle), original.ctx.handle)
+    res = avg_plaquette(expanded), check_constrained_plaquette(expanded, mag)
+    if expanded_config is not None:
+        export_config_nersc(expanded, expanded_config, Arg_export(0.0, 0, "lattice expansion"), does_skip_third=True)
+    return res
 
 
 # ------------------------------------------------------------ NERSC I/O

@@ -89,6 +89,10 @@ SIGNATURES = {
     "mag_kinetic_energy": [_P, C.c_int, _D],
     "mag_hamiltonian": [_P, _P, C.POINTER(Action), _D, _D],
     "mag_avg_wilson_loop": [_P, _I, C.c_int, _D],
+    "mag_total_plaq_async": [_P, _P],
+    "mag_total_rectangular_async": [_P, _P],
+    "mag_hamiltonian_async": [_P, _P, C.POINTER(Action), _P],
+    "mag_avg_wilson_loop_async": [_P, _I, C.c_int, _P],
     "mag_chain_create": [_P, _I, C.POINTER(Action), C.POINTER(HmcParams), C.POINTER(_P)],
     "mag_chain_destroy": [_P],
     "mag_chain_set_links_aos": [_P, _D],
@@ -100,6 +104,10 @@ SIGNATURES = {
     "mag_chain_demon_state": [_P, _D],
     "mag_chain_traj_index": [_P, _L],
     "mag_demon_microcanonical": [_P, C.c_int, C.c_double, C.c_uint64, _D, _L],
+    "mag_coarsen_links": [_P, C.c_int, _P],
+    "mag_update_constrain": [_P, _P, C.c_int],
+    "mag_constrained_plaquette": [_P, C.c_int, _D],
+    "mag_lattice_expansion": [_P, C.c_int, _P],
     "mag_nersc_write": [C.c_char_p, _D, C.POINTER(NerscHeader)],
     "mag_nersc_read": [C.c_char_p, _D, C.POINTER(NerscHeader)],
     "mag_export_config_nersc": [_P, C.c_char_p, C.c_double, C.c_int, C.c_char_p, C.c_int],

@@ -133,12 +133,14 @@ int mag_ctx_create(int device, mag_ctx* out) {
 int mag_ctx_destroy(mag_ctx c) {
   if (!c) return MAG_ERR_ARG;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  // a context on a borrowed stream (serving) is destroyed by its owner
+  // once its own work is known to be complete
+  if (c->owns_stream) (void)hipStreamSynchronize(c->stream);
   collect_timers(c);
   c->ws.clear();
-  if (c->d_part) (void)hipFree(c->d_part);
-  if (c->d_scalar) (void)hipFree(c->d_scalar);
-  if (c->h_scalar) (void)hipHostFree(c->h_scalar);
+  dev_free(c, c->d_part, c->part_cap * sizeof(double));
+  dev_free(c, c->d_scalar, 16 * sizeof(double));
+  host_free(c, c->h_scalar, 16 * sizeof(double));
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->owns_stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -215,7 +217,8 @@ int mag_field_create(mag_ctx c, const int dims[4], int kind, int prec, mag_field
 int mag_field_destroy(mag_field f) {
   if (!f) return MAG_ERR_ARG;
   if (f->owned) {
-    (void)hipStreamSynchronize(f->ctx->stream);
+    // pooled (serving) memory is returned only after its chain's work completed
+    if (!f->ctx->pool) (void)hipStreamSynchronize(f->ctx->stream);
     dev_free(f->ctx, f->p, f->bytes);
   }
   delete f;
@@ -447,6 +450,78 @@ int mag_avg_wilson_loop(mag_field U, const int r3[3], int T, double* out) {
   if (r || (r = fetch_scalars(c, 4))) return r;
   if (out) *out = c->h_scalar[3] / (nperm * (double)U->g.V);
   return MAG_OK;
+}
+
+// ------------------------------------------------- asynchronous reductions
+// The same sums left on the device: d_out is device memory, written in the
+// context's stream order; nothing synchronises.
+}  // extern "C"
+
+namespace {
+// out[i] = sum_j w[i][j] s[j] over the context's reduction slots, written to
+// caller memory in stream order.
+struct LinCombArgs {
+  const double* s;
+  double* out;
+  int nout;
+  double w[3][4];
+};
+__global__ void k_lincomb(LinCombArgs a) {
+  const int i = threadIdx.x;
+  if (i < a.nout) a.out[i] = a.w[i][0] * a.s[0] + a.w[i][1] * a.s[1] + a.w[i][2] * a.s[2] + a.w[i][3] * a.s[3];
+}
+
+int lincomb(mag_ctx c, double* d_out, int nout, const double w[3][4]) {
+  if (!d_out) return MAG_ERR_ARG;
+  LinCombArgs a{c->d_scalar, d_out, nout, {}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 4; ++j) a.w[i][j] = w[i][j];
+  hipLaunchKernelGGL(k_lincomb, dim3(1), dim3(64), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return MAG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mag_total_plaq_async(mag_field U, double* d_out) {
+  if (!is_link(U) || !d_out) return MAG_ERR_ARG;
+  mag_action thin{MAG_WILSON, 0., 0., 0, 0, 0.};
+  int r = sums(U, thin, false);
+  const double w[3][4] = {{1., 0., 0., 0.}};
+  return r ? r : lincomb(U->ctx, d_out, 1, w);
+}
+
+int mag_total_rectangular_async(mag_field U, double* d_out) {
+  if (!is_link(U) || !d_out) return MAG_ERR_ARG;
+  mag_action thin{MAG_IWASAKI, 0., 0., 0, 0, 0.};
+  int r = sums(U, thin, true);
+  const double w[3][4] = {{0., 1., 0., 0.}};
+  return r ? r : lincomb(U->ctx, d_out, 1, w);
+}
+
+int mag_hamiltonian_async(mag_field U, mag_field P, const mag_action* act, double* d_out) {
+  if (!is_link(U) || !is_alg(P) || !act || !d_out || !same_geom(U, P) || U->ctx != P->ctx) return MAG_ERR_ARG;
+  mag_ctx c = U->ctx;
+  int r = sums(U, *act, act->type == MAG_IWASAKI);
+  if (r || (r = kinetic_sum(c, P->g, P->prec, P->p, act->mag, 2))) return r;
+  // S = wp Sp + wr Sr (field-hmc.h:632-640), K = sum / 2
+  const double wp = action_from_sums(*act, 1., 0.), wr = action_from_sums(*act, 0., 1.);
+  const double w[3][4] = {{0., 0., 0.5, 0.}, {wp, wr, 0., 0.}, {wp, wr, 0.5, 0.}};
+  return lincomb(c, d_out, 3, w);
+}
+
+int mag_avg_wilson_loop_async(mag_field U, const int r3[3], int T, double* d_out) {
+  if (!is_link(U) || !r3 || T <= 0 || !d_out) return MAG_ERR_ARG;
+  for (int i = 0; i < 3; ++i)
+    if (r3[i] < 0) return MAG_ERR_ARG;
+  mag_ctx c = U->ctx;
+  int nperm = 1;
+  int r = dispatch_link(c, U->prec, U->kind, [&](auto t) -> int {
+    return Ops<typename decltype(t)::T, decltype(t)::N>::wilson_loop(c, U->g, U->p, r3, T, 3, &nperm);
+  });
+  const double w[3][4] = {{0., 0., 0., 1. / (nperm * (double)U->g.V)}};
+  return r ? r : lincomb(c, d_out, 1, w);
 }
 
 }  // extern "C"

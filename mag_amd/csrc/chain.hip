@@ -122,18 +122,19 @@ int chain_init_demon(mag_chain c) {
   return MAG_OK;
 }
 
-int ctx_child_create(mag_ctx parent, hipStream_t stream, mag_ctx* out) {
+int ctx_child_create(mag_ctx parent, hipStream_t stream, std::shared_ptr<MemPool> pool, mag_ctx* out) {
   auto* c = new mag_ctx_s();
   c->device = parent->device;
   c->stream = stream;
   c->owns_stream = false;
-  if (hipMalloc(&c->d_scalar, 16 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc(&c->h_scalar, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    if (c->d_scalar) (void)hipFree(c->d_scalar);
+  c->pool = std::move(pool);
+  if (dev_alloc(c, (void**)&c->d_scalar, 16 * sizeof(double)) ||
+      host_alloc(c, (void**)&c->h_scalar, 16 * sizeof(double))) {
+    dev_free(c, c->d_scalar, 16 * sizeof(double));
+    parent->err = c->err;
     delete c;
-    return fail(parent, MAG_ERR_OOM, "child context allocation failed");
+    return MAG_ERR_OOM;
   }
-  c->bytes = 16 * sizeof(double);
   *out = c;
   return MAG_OK;
 }

@@ -40,7 +40,7 @@ bool chain_needs_rect(mag_chain c);
 
 // A context sharing `parent`'s device and the given stream, with its own
 // workspace, partial-sum and scalar buffers.
-int ctx_child_create(mag_ctx parent, hipStream_t stream, mag_ctx* out);
+int ctx_child_create(mag_ctx parent, hipStream_t stream, std::shared_ptr<MemPool> pool, mag_ctx* out);
 // Footprint of a chain in device bytes (fields + workspace), for admission.
 size_t chain_footprint(const int dims[4], const mag_action& act, const mag_hmc_params& p);
 

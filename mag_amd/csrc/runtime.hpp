@@ -32,6 +32,32 @@ struct TimerRec {
   long launches = 0;
 };
 
+// Device and pinned-host blocks of the chains a server retires, kept for the
+// chains it admits next instead of freed: hipFree and hipHostFree wait for
+// the whole device, which would stall every batch in flight each time a
+// chain retires. Blocks are reused by exact size (a serving stream repeats
+// a handful of lattice shapes); when an allocation fails the cached blocks
+// are released and the allocation retried.
+struct MemPool {
+  std::multimap<size_t, void*> dev, host;
+  size_t dev_cached = 0;
+  ~MemPool() { release(); }
+  void* take(std::multimap<size_t, void*>& m, size_t bytes) {
+    auto it = m.find(bytes);
+    if (it == m.end()) return nullptr;
+    void* p = it->second;
+    m.erase(it);
+    return p;
+  }
+  void release() {
+    for (auto& kv : dev) (void)hipFree(kv.second);
+    for (auto& kv : host) (void)hipHostFree(kv.second);
+    dev.clear();
+    host.clear();
+    dev_cached = 0;
+  }
+};
+
 }  // namespace mag
 
 struct mag_ctx_s {
@@ -49,6 +75,7 @@ struct mag_ctx_s {
   std::vector<hipEvent_t> event_pool;
   mag::Recorder* rec = nullptr;  // set while a serving batch records its launches
   bool owns_stream = true;
+  std::shared_ptr<mag::MemPool> pool;  // serving child contexts: frees go back to the server's pool
 };
 
 struct mag_field_s {
@@ -128,7 +155,19 @@ inline void collect_timers(mag_ctx c) {
 }
 
 inline int dev_alloc(mag_ctx c, void** p, size_t bytes) {
-  if (hipMalloc(p, bytes) != hipSuccess) {
+  *p = nullptr;
+  if (c->pool && (*p = c->pool->take(c->pool->dev, bytes))) {
+    c->pool->dev_cached -= bytes;
+    c->bytes += bytes;
+    return MAG_OK;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && c->pool && c->pool->dev_cached) {  // memory held by the pool: give it back, retry
+    (void)hipGetLastError();
+    c->pool->release();
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
     return fail(c, MAG_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
@@ -137,10 +176,28 @@ inline int dev_alloc(mag_ctx c, void** p, size_t bytes) {
   return MAG_OK;
 }
 inline void dev_free(mag_ctx c, void* p, size_t bytes) {
-  if (p) {
+  if (!p) return;
+  if (c->pool) {
+    c->pool->dev.emplace(bytes, p);
+    c->pool->dev_cached += bytes;
+  } else {
     (void)hipFree(p);
-    c->bytes -= bytes;
   }
+  c->bytes -= bytes;
+}
+inline int host_alloc(mag_ctx c, void** p, size_t bytes) {
+  if (c->pool && (*p = c->pool->take(c->pool->host, bytes))) return MAG_OK;
+  if (hipHostMalloc(p, bytes, hipHostMallocDefault) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, MAG_ERR_OOM, "pinned host allocation failed");
+  }
+  return MAG_OK;
+}
+inline void host_free(mag_ctx c, void* p, size_t bytes) {
+  if (!p) return;
+  if (c->pool) c->pool->host.emplace(bytes, p);
+  else (void)hipHostFree(p);
 }
 
 inline int ensure_part(mag_ctx c, size_t n) {

@@ -20,6 +20,9 @@
 //     uploads and graph capture run on the context's own stream, and every
 //     host->device copy of a launch comes from pinned memory, so launching
 //     never blocks the host;
+//   - memory: chains and batches draw device and pinned-host blocks from a
+//     pool owned by the server and return them to it; nothing is freed
+//     while the server runs (hipFree would wait for the whole device);
 //   - completion: a finished batch's FP64 scalars (one copy per batch) feed
 //     the accept decisions on the host; rejected chains are restored on the
 //     batch's stream; finished, cancelled and non-finite chains retire, and
@@ -62,6 +65,11 @@ struct Entry {
   std::vector<double> t_done;
   bool cancel = false;
   Batch* batch = nullptr;  // batch the chain belongs to (null: not yet batched)
+  // Recorded on the chain's stream after its accept step (which may restore
+  // the links with a copy): its next trajectory, on whatever stream its batch
+  // is launched next, waits for it.
+  hipEvent_t ready = nullptr;
+  bool ready_recorded = false;
 };
 
 using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
@@ -112,6 +120,8 @@ struct mag_server_s {
   bool graphs = true;
   hipStream_t streams[NSTREAMS] = {};
   hipStream_t aux = nullptr;  // the context's stream: admission, table uploads, graph capture
+  std::shared_ptr<MemPool> pool = std::make_shared<MemPool>();
+  mag_ctx mem = nullptr;  // pool-backed context for the batches' own buffers
   double load[NSTREAMS] = {};
   std::vector<std::unique_ptr<Entry>> entries;
   std::deque<Entry*> pending;
@@ -132,10 +142,10 @@ void release_tables(mag_server s, Batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
-  if (b->tables) dev_free(s->ctx, b->tables, b->table_bytes);
-  if (b->gather) dev_free(s->ctx, b->gather, b->gather_n * S_COUNT * sizeof(double));
-  if (b->h_gather) (void)hipHostFree(b->h_gather);
-  if (b->h_epoch) (void)hipHostFree(b->h_epoch);
+  dev_free(s->mem, b->tables, b->table_bytes);
+  dev_free(s->mem, b->gather, b->gather_n * S_COUNT * sizeof(double));
+  host_free(s->mem, b->h_gather, b->gather_n * S_COUNT * sizeof(double));
+  host_free(s->mem, b->h_epoch, b->gather_n * sizeof(uint64_t));
   b->tables = nullptr;
   b->gather = nullptr;
   b->gather_n = 0;
@@ -161,11 +171,11 @@ int build_batch(mag_server s, Batch* b) {
   release_tables(s, b);
   if (!b->done && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
     return fail(x, MAG_ERR_DEVICE, "event creation failed");
-  if ((r = dev_alloc(x, (void**)&b->gather, n * S_COUNT * sizeof(double)))) return r;
   b->gather_n = n;
-  if (hipHostMalloc(&b->h_gather, n * S_COUNT * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&b->h_epoch, n * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
-    return fail(x, MAG_ERR_OOM, "pinned gather buffer");
+  if ((r = dev_alloc(s->mem, (void**)&b->gather, n * S_COUNT * sizeof(double))) ||
+      (r = host_alloc(s->mem, (void**)&b->h_gather, n * S_COUNT * sizeof(double))) ||
+      (r = host_alloc(s->mem, (void**)&b->h_epoch, n * sizeof(uint64_t))))
+    return fail(x, r, s->mem->err);
   b->rec = Recorder();
   for (size_t i = 0; i < n; ++i) {
     mag_chain c = b->members[i]->chain;
@@ -181,7 +191,7 @@ int build_batch(mag_server s, Batch* b) {
   // argument tables, one device block
   size_t total = 0;
   for (auto& c : b->rec.calls) total += (c.blob.size() + 255) / 256 * 256;
-  if ((r = dev_alloc(x, &b->tables, total))) return r;
+  if ((r = dev_alloc(s->mem, &b->tables, total))) return fail(x, r, s->mem->err);
   b->table_bytes = total;
   std::vector<char> host(total);
   size_t off = 0;
@@ -215,7 +225,10 @@ int launch_batch(mag_server s, Batch* b) {
   if (sites < LARGE_SITES) best = (s->load[1] <= s->load[2]) ? 1 : 2;
   b->sidx = best;
   hipStream_t st = s->streams[best];
-  for (Entry* e : b->members) e->chain->ctx->stream = st;
+  for (Entry* e : b->members) {
+    if (e->ready_recorded) HIPCHK(x, hipStreamWaitEvent(st, e->ready, 0));
+    e->chain->ctx->stream = st;
+  }
   // trajectory index of every member (read by the momentum kernel), from
   // pinned memory: the copies are asynchronous
   for (size_t i = 0; i < n; ++i) {
@@ -250,6 +263,9 @@ int retire(mag_server s, Entry* e, int state) {
       return fail(s->ctx, MAG_ERR_DEVICE, "retire event failed");
     s->graves.push_back({e->chain, ev, e->footprint});
     e->chain = nullptr;
+    if (e->ready) (void)hipEventDestroy(e->ready);
+    e->ready = nullptr;
+    e->ready_recorded = false;
   } else {
     s->used -= e->footprint;
   }
@@ -296,7 +312,7 @@ int admit(mag_server s) {
     if (s->used + e->footprint > s->budget) break;  // arrival order: wait for memory
     s->pending.pop_front();
     mag_ctx cc = nullptr;
-    int r = ctx_child_create(s->ctx, s->aux, &cc);
+    int r = ctx_child_create(s->ctx, s->aux, s->pool, &cc);
     if (r) return r;
     mag_chain ch = nullptr;
     r = mag_chain_create(cc, e->req.dims, &e->req.act, &e->req.hmc, &ch);
@@ -308,6 +324,8 @@ int admit(mag_server s) {
     }
     ch->owns_ctx = true;
     e->chain = ch;
+    if (hipEventCreateWithFlags(&e->ready, hipEventDisableTiming) != hipSuccess)
+      return fail(s->ctx, MAG_ERR_DEVICE, "event creation failed");
     if ((r = mag_haar_links(ch->U, e->req.links_seed))) return r;
     if (e->req.hmc.accept_mode != 0 && (r = chain_init_demon(ch))) return r;
     s->used += e->footprint;
@@ -336,7 +354,12 @@ int complete(mag_server s, Batch* b) {
     if (res.status != MAG_OK) st = ST_FAILED;
     else if (e->cancel) st = ST_CANCELLED;
     else if ((int)e->results.size() >= e->req.ntraj) st = ST_DONE;
-    if (st >= 0 && (r = retire(s, e, st))) return r;
+    if (st >= 0) {
+      if ((r = retire(s, e, st))) return r;
+    } else {
+      HIPCHK(s->ctx, hipEventRecord(e->ready, e->chain->ctx->stream));
+      e->ready_recorded = true;
+    }
   }
   return MAG_OK;
 }
@@ -453,6 +476,10 @@ int mag_server_create(mag_ctx ctx, size_t hbm_budget, int max_batch, int use_gra
       return fail(ctx, MAG_ERR_DEVICE, "stream creation failed");
     }
   s->aux = ctx->stream;
+  if (ctx_child_create(ctx, s->aux, s->pool, &s->mem)) {
+    mag_server_destroy(s);
+    return MAG_ERR_OOM;
+  }
   *out = s;
   return MAG_OK;
 }
@@ -468,6 +495,9 @@ int mag_server_destroy(mag_server s) {
     if (e->chain) (void)retire(s, e.get(), ST_CANCELLED);
   }
   (void)reap(s, true);
+  if (s->mem) mag_ctx_destroy(s->mem);
+  s->batches.clear();
+  s->pool.reset();  // the last chain context is gone: cached blocks are freed here
   for (auto& st : s->streams)
     if (st) (void)hipStreamDestroy(st);
   delete s;

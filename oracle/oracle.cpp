@@ -1075,6 +1075,102 @@ void orc_demon_microcanonical(const int* L, double* U, int nsweeps, double E_max
   acc_rej[1] = rej;
 }
 
+// ------------------------------------------------- mag-blocked lattices
+// Coarse link = ordered product of the mag fine links from mag*xC along mu:
+// the coarse-field initialisation of double_multigrid (field-hmc.h:1068-1078).
+void orc_coarsen(const int* Lf, const double* Uf, int mag, double* Uc) {
+  Geo gf(Lf);
+  int Lc[4];
+  for (int i = 0; i < 4; ++i) Lc[i] = Lf[i] / mag;
+  Geo gc(Lc);
+  std::vector<Mat> u = load_field(gf, Uf), c(gc.V * 4);
+  Field f{&gf, u.data()};
+  std::vector<int> dir(mag);
+  for (long s = 0; s < gc.V; ++s) {
+    int xc[4], x[4];
+    gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = mag * xc[i];
+    for (int mu = 0; mu < 4; ++mu) {
+      std::fill(dir.begin(), dir.end(), mu);
+      c[s * 4 + mu] = path_product(f, x, dir.data(), mag);
+    }
+  }
+  store_field(c, Uc);
+}
+
+// update_constrain (field-hmc.h:988-1009): the first fine link of every
+// coarse link becomes U_c(xC, mu) [U(x+mu) ... U(x+(mag-1)mu)]^+, so the
+// product of the mag fine links equals the coarse link.
+void orc_update_constrain(const int* Lf, double* Uf, const double* Uc, int mag) {
+  Geo gf(Lf);
+  int Lc[4];
+  for (int i = 0; i < 4; ++i) Lc[i] = Lf[i] / mag;
+  Geo gc(Lc);
+  std::vector<Mat> u = load_field(gf, Uf), c = load_field(gc, Uc);
+  Field f{&gf, u.data()};
+  std::vector<int> dir(mag > 1 ? mag - 1 : 1);
+  for (long s = 0; s < gc.V; ++s) {
+    int xc[4], x[4];
+    gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = mag * xc[i];
+    for (int mu = 0; mu < 4; ++mu) {
+      int xp[4] = {x[0], x[1], x[2], x[3]};
+      xp[mu]++;
+      std::fill(dir.begin(), dir.end(), mu);
+      const Mat m = path_product(f, xp, dir.data(), mag - 1);
+      f.at(x, mu) = c[s * 4 + mu] * dagger(m);
+    }
+  }
+  store_field(u, Uf);
+}
+
+// check_constrained_plaquette (field-matrix.h:407-443): Re Tr / 3 of the
+// mag x mag loops at the coarse sites, all planes nu < mu, averaged.
+double orc_constrained_plaquette(const int* L, const double* U, int mag) {
+  Geo g(L);
+  std::vector<Mat> u = load_field(g, U);
+  Field f{&g, u.data()};
+  long count = 0;
+  double sum = 0.;
+  std::vector<int> dir(4 * mag);
+  for (int x0 = 0; x0 < L[0]; x0 += mag)
+    for (int x1 = 0; x1 < L[1]; x1 += mag)
+      for (int x2 = 0; x2 < L[2]; x2 += mag)
+        for (int x3 = 0; x3 < L[3]; x3 += mag) {
+          const int x[4] = {x0, x1, x2, x3};
+          for (int mu = 0; mu < 4; ++mu)
+            for (int nu = 0; nu < mu; ++nu) {
+              for (int i = 0; i < mag; ++i) {
+                dir[i] = mu;
+                dir[mag + i] = nu;
+                dir[2 * mag + i] = mu + DIMN;
+                dir[3 * mag + i] = nu + DIMN;
+              }
+              sum += retr(path_product(f, x, dir.data(), 4 * mag));
+              ++count;
+            }
+        }
+  return sum / (3. * count);
+}
+
+// lattice_expansion (expansion/main.C:19-62) without the file I/O: unit links
+// everywhere, the coarse link U(x, mu) at fine site mag*x.
+void orc_expand(const int* Lc, const double* Uc, int mag, double* Uf) {
+  Geo gc(Lc);
+  int Lf[4];
+  for (int i = 0; i < 4; ++i) Lf[i] = Lc[i] * mag;
+  Geo gf(Lf);
+  std::vector<Mat> c = load_field(gc, Uc), u(gf.V * 4, Mat::unit());
+  Field f{&gf, u.data()};
+  for (long s = 0; s < gc.V; ++s) {
+    int xc[4], x[4];
+    gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = mag * xc[i];
+    for (int mu = 0; mu < 4; ++mu) f.at(x, mu) = c[s * 4 + mu];
+  }
+  store_field(u, Uf);
+}
+
 int orc_num_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -85,6 +85,13 @@ int orc_run_hmc(const int* L, double* U, const orc_action* a, const orc_hmc* h, 
 void orc_demon_microcanonical(const int* L, double* U, int nsweeps, double E_max, uint64_t seed,
                               double* E /* [4] */, long* acc_rej /* [2] */);
 
+/* mag-blocked lattices: double_multigrid coarse init (field-hmc.h:1068-1078),
+ * update_constrain (field-hmc.h:988-1009), check_constrained_plaquette
+ * (field-matrix.h:407-443), lattice_expansion (expansion/main.C:19-62) */
+void orc_coarsen(const int* Lf, const double* Uf, int mag, double* Uc);
+void orc_update_constrain(const int* Lf, double* Uf, const double* Uc, int mag);
+double orc_constrained_plaquette(const int* L, const double* U, int mag);
+void orc_expand(const int* Lc, const double* Uc, int mag, double* Uf);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -91,6 +91,10 @@ def _declare(l):
                                   C.POINTER(Traj)]),
         "orc_demon_microcanonical": (None, [_I, _D, C.c_int, C.c_double, C.c_uint64, _D,
                                             C.POINTER(C.c_long)]),
+        "orc_coarsen": (None, [_I, _D, C.c_int, _D]),
+        "orc_update_constrain": (None, [_I, _D, _D, C.c_int]),
+        "orc_constrained_plaquette": (C.c_double, [_I, _D, C.c_int]),
+        "orc_expand": (None, [_I, _D, C.c_int, _D]),
         "orc_num_threads": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -304,6 +308,32 @@ def demon_microcanonical(L, U, nsweeps, E_max, seed, E=None):
     ar = (C.c_long * 2)()
     lib().orc_demon_microcanonical(_dims(L), _dptr(U), nsweeps, E_max, seed, _dptr(E), ar)
     return E, (ar[0], ar[1])
+
+
+def coarsen(L, U, mag):
+    """Coarse links of a mag-blocked lattice (double_multigrid, field-hmc.h:1068-1078)."""
+    Lc = [v // mag for v in L]
+    out = np.zeros((volume(Lc), 4, 18))
+    lib().orc_coarsen(_dims(L), _dptr(U), mag, _dptr(out))
+    return out
+
+
+def update_constrain(L, U, Uc, mag):
+    """update_constrain (field-hmc.h:988-1009), in place on U."""
+    lib().orc_update_constrain(_dims(L), _dptr(U), _dptr(Uc), mag)
+
+
+def constrained_plaquette(L, U, mag):
+    """check_constrained_plaquette (field-matrix.h:407-443)."""
+    return lib().orc_constrained_plaquette(_dims(L), _dptr(U), mag)
+
+
+def expand(Lc, Uc, mag):
+    """lattice_expansion (expansion/main.C:19-62) without file I/O."""
+    Lf = [v * mag for v in Lc]
+    out = np.zeros((volume(Lf), 4, 18))
+    lib().orc_expand(_dims(Lc), _dptr(Uc), mag, _dptr(out))
+    return out
 
 
 def num_threads():

@@ -411,3 +411,35 @@ def test_c5_shape_64x4_dbw2_3stout_fp32(ctx):
     w64 = M.avg_wilson_loop(U64, (4, 0, 0), 4)
     assert np.isfinite(w32) and abs(w32 - w64) < 1e-12
     assert abs(M.avg_plaquette(U64) - r.avg_plaq) < 1e-12
+
+
+def test_async_reductions_match_synchronous(ctx, orc):
+    """The _async reductions leave their results in device memory, in stream
+    order. Read back, the sums equal the synchronous calls bit for bit; the
+    potential and H, which the device combines from the same sums with
+    fused multiply-adds, agree to rounding. (Device memory for the results
+    comes straight from the HIP runtime.)"""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    ptr = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(64)) == 0
+    try:
+        U = links(ctx, L8, orc.haar_links(L8, 3))
+        P = alg(ctx, L8)
+        M.init_momentum(P, 5)
+        arg = M.Arg_chmc(beta=2.6, gauge=M.Gauge(M.IWASAKI, -0.331))
+        base = ptr.value
+        M.total_plaq_async(U, base)
+        M.total_rectangular_async(U, base + 8)
+        M.get_hamiltonian_async(U, P, arg, base + 16)
+        M.avg_wilson_loop_async(U, (2, 1, 0), 2, base + 40)
+        ctx.synchronize()
+        h = np.zeros(8)
+        assert hip.hipMemcpy(h.ctypes.data_as(ctypes.c_void_p), ptr, ctypes.c_size_t(48), 2) == 0  # D2H
+        H, parts = M.get_hamiltonian(U, P, arg)
+        assert h[0] == M.total_plaq(U) and h[1] == M.total_rectangular(U)
+        assert h[2] == parts[0]
+        assert abs(h[3] - parts[1]) <= 1e-14 * abs(parts[1]) and abs(h[4] - H) <= 1e-14 * abs(H)
+        assert h[5] == M.avg_wilson_loop(U, (2, 1, 0), 2)
+    finally:
+        hip.hipFree(ptr)

@@ -414,3 +414,29 @@ def test_run_hmc_bookkeeping(orc):
         if i >= 2:
             assert r.accept == int(r.die_roll < np.exp(r.h_old - r.h_new))
     assert abs(res[-1].avg_plaq - orc.avg_plaquette(L, U)) < 1e-15
+
+
+# ------------------------------------------------------ mag-blocked lattices
+def test_coarse_fine_identities(orc):
+    """Pins of the coarse/fine tools by identities a mistake in a path, a
+    direction or a dagger breaks: update_constrain makes the fine products
+    equal the coarse links; the constrained plaquette of a fine field is the
+    plaquette of its coarse links; an expanded lattice's constrained
+    plaquette is the coarse plaquette; mag = 1 is the ordinary plaquette."""
+    Lf, Lc = (8, 8, 4, 8), (4, 4, 2, 4)
+    U = orc.haar_links(Lf, 3)
+    Uc = orc.haar_links(Lc, 9)
+    V = U.copy()
+    orc.update_constrain(Lf, V, Uc, 2)
+    assert np.abs(orc.coarsen(Lf, V, 2) - Uc).max() < 1e-13
+    changed = np.abs(V - U).max(axis=2)
+    origins = np.zeros((8, 4, 8, 8), bool)
+    origins[::2, ::2, ::2, ::2] = True  # [x3, x2, x1, x0]
+    assert not changed.reshape(8, 4, 8, 8, 4)[~origins].any()  # only the first link of each coarse link moves
+    assert abs(orc.constrained_plaquette(Lf, U, 2) - orc.avg_plaquette(Lc, orc.coarsen(Lf, U, 2))) < 1e-14
+    E = orc.expand(Lc, Uc, 2)
+    assert abs(orc.constrained_plaquette(Lf, E, 2) - orc.avg_plaquette(Lc, Uc)) < 1e-14
+    assert abs(orc.constrained_plaquette(Lf, U, 1) - orc.avg_plaquette(Lf, U)) < 1e-14
+    unit = orc.unit_links(Lf)
+    assert orc.avg_plaquette(Lf, E) != orc.avg_plaquette(Lc, Uc)  # the expansion is not the coarse lattice
+    assert abs(orc.constrained_plaquette(Lf, unit, 2) - 1.0) < 1e-15
