@@ -274,6 +274,10 @@ int mag_server_status(mag_server s, int id, int* state, int* done_traj);
 /* per-trajectory results and completion times (seconds since server creation) */
 int mag_server_results(mag_server s, int id, mag_traj_result* out, double* t_done, int max);
 int mag_server_info(mag_server s, size_t* used_bytes, long* trajectories, double* elapsed_s);
+/* host seconds spent in admission, batch builds (recording, table upload,
+ * graph capture), launches, waiting for completions and accept/retire
+ * processing; builds = number of batch (re)builds */
+int mag_server_host_times(mag_server s, double seconds[5], long* builds);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,7 @@ SIGNATURES = {
     "mag_server_status": [_P, C.c_int, _I, _I],
     "mag_server_results": [_P, C.c_int, C.POINTER(TrajResult), _D, C.c_int],
     "mag_server_info": [_P, C.POINTER(C.c_size_t), _L, _D],
+    "mag_server_host_times": [_P, _D, _L],
 }
 
 

@@ -129,6 +129,10 @@ struct mag_server_s {
   std::vector<Grave> graves;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   long trajectories = 0;
+  // host seconds spent in: admission, batch builds (record, upload, graph
+  // capture), launches, waiting for completions, accept/retire processing
+  double host_s[5] = {0., 0., 0., 0., 0.};
+  long builds = 0;
 };
 
 namespace {
@@ -424,7 +428,14 @@ int launch_idle(mag_server s) {
   s->batches.swap(keep);
   for (auto& b : s->batches) {
     if (b->inflight) continue;
-    if (!b->valid && (r = build_batch(s, b.get()))) return r;
+    if (!b->valid) {
+      const double t0 = now_s(s);
+      r = build_batch(s, b.get());
+      s->host_s[1] += now_s(s) - t0;
+      s->host_s[2] -= now_s(s) - t0;  // counted as build, not launch
+      ++s->builds;
+      if (r) return r;
+    }
     if ((r = launch_batch(s, b.get()))) return r;
   }
   return MAG_OK;
@@ -433,6 +444,7 @@ int launch_idle(mag_server s) {
 // Waits until at least one batch in flight has completed (if any is), then
 // completes every batch that has.
 int poll(mag_server s) {
+  double tw = now_s(s);
   for (;;) {
     bool inflight = false, any = false;
     for (size_t i = 0; i < s->batches.size(); ++i) {
@@ -444,11 +456,18 @@ int poll(mag_server s) {
         continue;
       }
       if (q != hipSuccess) return fail(s->ctx, MAG_ERR_DEVICE, "batch failed on the device");
+      const double tc = now_s(s);
+      s->host_s[3] += tc - tw;
       int r;
       if ((r = complete(s, b))) return r;
+      tw = now_s(s);
+      s->host_s[4] += tw - tc;
       any = true;
     }
-    if (any || !inflight) return MAG_OK;
+    if (any || !inflight) {
+      s->host_s[3] += now_s(s) - tw;
+      return MAG_OK;
+    }
     std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
 }
@@ -531,7 +550,14 @@ int mag_server_cancel(mag_server s, int id) {
 int mag_server_step(mag_server s, int* active, int* pending) {
   if (!s) return MAG_ERR_ARG;
   int r;
-  if ((r = reap(s, false)) || (r = admit(s)) || (r = launch_idle(s)) || (r = poll(s))) return r;
+  double t = now_s(s);
+  if ((r = reap(s, false)) || (r = admit(s))) return r;
+  double t1 = now_s(s);
+  s->host_s[0] += t1 - t;
+  if ((r = launch_idle(s))) return r;
+  t = now_s(s);
+  s->host_s[2] += t - t1;
+  if ((r = poll(s))) return r;
   int na = 0, np = 0;
   for (auto& e : s->entries) {
     na += e->state == ST_RUNNING;
@@ -559,6 +585,13 @@ int mag_server_results(mag_server s, int id, mag_traj_result* out, double* t_don
     if (out) out[i] = e->results[i];
     if (t_done) t_done[i] = e->t_done[i];
   }
+  return MAG_OK;
+}
+
+int mag_server_host_times(mag_server s, double seconds[5], long* builds) {
+  if (!s || !seconds) return MAG_ERR_ARG;
+  for (int i = 0; i < 5; ++i) seconds[i] = s->host_s[i];
+  if (builds) *builds = s->builds;
   return MAG_OK;
 }
 

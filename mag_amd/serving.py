@@ -93,6 +93,12 @@ class Server:
         _check(_lib.lib().mag_server_info(self._h, C.byref(u), C.byref(n), C.byref(e)), self.ctx.handle)
         return u.value, n.value, e.value
 
+    def host_times(self):
+        """Host seconds in admission, batch builds, launches, waiting, accept/retire; and the build count."""
+        t, n = (C.c_double * 5)(), C.c_long()
+        _check(_lib.lib().mag_server_host_times(self._h, t, C.byref(n)), self.ctx.handle)
+        return dict(zip(("admit", "build", "launch", "wait", "complete"), list(t))), n.value
+
     def run_until_idle(self, max_steps=100000):
         for _ in range(max_steps):
             a, p = self.step()
