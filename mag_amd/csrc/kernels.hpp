@@ -4,11 +4,12 @@
 // registers); only the site coordinates are per-lane.
 //
 // Streaming kernels take blockIdx.x = site block, blockIdx.y = mu. Stencil
-// kernels (staples, stout levels) take a 1D grid of 4 * nb tiles, tile =
-// 4 * site block + mu, remapped by stencil_tile() so that each of the 8 XCDs
-// walks one contiguous slab of the lattice and dispatches the four
-// directions of a site block back to back: the neighbour links all four
-// read are then fetched into that XCD's L2 once.
+// kernels (staples, stout levels) take a 1D grid of 4 * nb tiles (one per
+// 256-site block and direction), remapped by stencil_tile() so that each of
+// the 8 XCDs (blocks go to XCDs round-robin) works on its own slab of the
+// lattice, the four directions of a site block are dispatched back to back,
+// and the neighbours of the tiles in flight are in flight or just done: the
+// links they share are then fetched into that XCD's L2 once.
 //
 // Template parameters: TS storage real, NC reals per stored link (12 or 18),
 // TC compute real. FP64 kernels (TS = TC = double, NC = 18) follow the
@@ -35,9 +36,29 @@ constexpr int stencil_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
 #define MAG_STENCIL_OCC __attribute__((amdgpu_waves_per_eu(stencil_waves<TC>())))
 
 // Blocks are dispatched round-robin over the 8 XCDs (XCD = linear block id
-// mod 8); XCD k gets tiles [k nb/8, (k+1) nb/8) in order.
-MAG_D void stencil_tile(int& mu, int& sblk) {
+// mod 8). When an (x0, x1) plane is a whole number of blocks and L3 splits
+// into 8 slabs, XCD k owns x3 in [k L3/8, (k+1) L3/8) and walks it with the
+// direction fastest, then x3, then the (x0, x1) block, then x2: the tiles in
+// flight on one XCD (about 100) then cover whole (x0, x1, x3) slabs at one
+// or two values of x2, so their x3 neighbours are in flight and their x2
+// neighbours were just done. Otherwise XCD k takes tiles [k nb/8, (k+1) nb/8)
+// in lexicographic order.
+MAG_D void stencil_tile(const Geom& g, int& mu, int& sblk) {
   const int nb = gridDim.x, b = blockIdx.x;
+  const int plane = g.L[0] * g.L[1];
+  if (nb % 8 == 0 && plane % BLOCK == 0 && g.L[3] % 8 == 0 && nb == 4 * (g.V / BLOCK)) {
+    const int xcd = b % 8, j = b / 8;
+    const int h3 = g.L[3] / 8, nb01 = plane / BLOCK;
+    mu = j & 3;
+    int r = j >> 2;
+    const int dx3 = r % h3;
+    r /= h3;
+    const int b01 = r % nb01;
+    const int x2 = r / nb01;
+    const int x3 = xcd * h3 + dx3;
+    sblk = (x3 * g.L[2] + x2) * nb01 + b01;
+    return;
+  }
   const int t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
   mu = t & 3;
   sblk = t >> 2;
@@ -184,7 +205,7 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 template <int ACT, class TS, int NC, class TC>
 MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   int mu, sblk;
-  stencil_tile(mu, sblk);
+  stencil_tile(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
@@ -305,7 +326,7 @@ struct StoutFwdArgs {
 template <class TSI, int NCI, class TSO, int NCO, class TC>
 MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
   int mu, sblk;
-  stencil_tile(mu, sblk);
+  stencil_tile(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const Site p = site_of(a.g, s);
@@ -368,7 +389,7 @@ struct StoutTopArgs {
 template <int ACT, class TS, int NC, class TC>
 MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
   int mu, sblk;
-  stencil_tile(mu, sblk);
+  stencil_tile(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
@@ -482,7 +503,7 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
 template <bool LAST, class TS, int NC, class TC>
 MAG_D void k_stout_bwd_body(const StoutBwdArgs<TS, NC>& a) {
   int mu, sblk;
-  stencil_tile(mu, sblk);
+  stencil_tile(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const Site p = site_of(a.g, s);

@@ -443,3 +443,28 @@ def test_async_reductions_match_synchronous(ctx, orc):
         assert h[5] == M.avg_wilson_loop(U, (2, 1, 0), 2)
     finally:
         hip.hipFree(ptr)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_fg_dh_scales_as_dt4(ctx, orc, prec):
+    """Force-gradient integration on the device: from a smooth configuration
+    (links e^{iQ} with |Q| ~ 0.3) and fixed momenta, halving the step at
+    tau = 1 divides dH by 2^4 (Iwasaki + 1-stout). FP32 MD holds the ratio
+    while dH stays well above its rounding noise."""
+    rng = np.random.default_rng(4)
+    T = orc.generators()
+    q = rng.normal(size=(np.prod(L8) * 4, 8)) * 0.3
+    U0 = orc.expiQ(np.einsum("na,aij->nij", q, T))
+    U0 = orc.as_real(U0).reshape(-1, 4, 18)
+    P0 = orc.init_momentum(L8, 77, 0)
+    dh = []
+    for n in (8, 16):
+        arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=5.0, gauge=M.Gauge(M.IWASAKI, -0.331), n_stout=1,
+                         rho=0.1)
+        U, P = links(ctx, L8, U0, prec), alg(ctx, L8, P0, prec)
+        h0, _ = M.get_hamiltonian(U, P, arg)
+        M.force_gradient_integrator(U, P, arg, reunit=False)
+        h1, _ = M.get_hamiltonian(U, P, arg)
+        dh.append(h1 - h0)
+    ratio = dh[0] / dh[1]
+    assert 13.0 < ratio < 19.0, dh
