@@ -294,8 +294,8 @@ MAG_HD void ch_lambda(const M3<T>& Q, const M3<T>& X, T lam[8], M3<T>& eQ) {
   const cpx<T> t1 = trace_mul(X, B1), t2 = trace_mul(X, B2);
   // Gamma = t1 Q + t2 Q^2 + f1 X + f2 (Q X + X Q)
   const cpx<T> g1 = c.f1, g2 = c.f2;
-  M3<T> QX = mul(Q, X);
-  mul_acc(QX, X, Q);
+  M3<T> QX = mul_hn(Q, X);
+  mul_acc_nh(QX, X, Q);
   M3<T> G;
 #pragma unroll
   for (int k = 0; k < 9; ++k) G.e[k] = cfma(t1, Q.e[k], cfma(t2, Q2.e[k], cfma(g1, X.e[k], g2 * QX.e[k])));

@@ -366,7 +366,7 @@ MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS,
   const M3<TC> Lm = alg_to_mat(lam);
   M3<TC> W = mul(Zp, eQ);
   // + i C^+ Lambda
-  const M3<TC> CL = mul_dn(Cv.template load<TC>(mu, s), Lm);
+  const M3<TC> CL = mul_dh(Cv.template load<TC>(mu, s), Lm);
 #pragma unroll
   for (int k = 0; k < 9; ++k) W.e[k] = W.e[k] + cpx<TC>{-CL.e[k].i, CL.e[k].r};
   Wout.store(mu, s, W);
@@ -472,12 +472,12 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
     const int xpn = p.s + opn, xmn = p.s + omn, xpmmn = xpm + omn;
     {
       const M3<TC> D = mul_nd(a.U.template load<TC>(nu, xpm), a.U.template load<TC>(mu, xpn));
-      M3<TC> G = mul(a.L.template load_mat<TC>(nu, xpm), D);
-      mul_sub(G, D, a.L.template load_mat<TC>(mu, xpn));
+      M3<TC> G = mul_hn(a.L.template load_mat<TC>(nu, xpm), D);
+      mul_sub_nh(G, D, a.L.template load_mat<TC>(mu, xpn));
       const M3<TC> Un_x = a.U.template load<TC>(nu, p.s);
       mul_nd_acc(acc, G, Un_x);
       const M3<TC> E = mul_nd(D, Un_x);
-      mul_sub(acc, E, a.L.template load_mat<TC>(nu, p.s));
+      mul_sub_nh(acc, E, a.L.template load_mat<TC>(nu, p.s));
     }
     {
       const M3<TC> Um_xmn = a.U.template load<TC>(mu, xmn);
@@ -486,8 +486,8 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
       a.L.load(mu, xmn, lm);
 #pragma unroll
       for (int k = 0; k < 8; ++k) ln[k] -= lm[k];
-      M3<TC> K = mul_dn(Um_xmn, alg_to_mat(ln));
-      mul_nd_sub(K, a.L.template load_mat<TC>(nu, xpmmn), Um_xmn);
+      M3<TC> K = mul_dh(Um_xmn, alg_to_mat(ln));
+      mul_hd_sub(K, a.L.template load_mat<TC>(nu, xpmmn), Um_xmn);
       const M3<TC> K2 = mul_dn(a.U.template load<TC>(nu, xpmmn), K);
       mul_acc(acc, K2, a.U.template load<TC>(nu, xmn));
     }

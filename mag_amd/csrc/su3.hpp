@@ -122,6 +122,85 @@ MAG_HD M3<T> mul_dn(const M3<T>& a, const M3<T>& b) {
     }
   return c;
 }
+// Products with a Hermitian factor H (Q, Lambda): the diagonal of H is real,
+// so three of the nine complex multiply-adds of every column (row) need only
+// half their instructions. Same terms in the same order as the general
+// products; the dropped ones are exact zeros.
+template <class T>
+MAG_HD cpx<T> rfma(T r, cpx<T> c, cpx<T> acc) { return {mfma(r, c.r, acc.r), mfma(r, c.i, acc.i)}; }
+template <class T>
+MAG_HD cpx<T> neg(cpx<T> a) { return {-a.r, -a.i}; }
+// C = H A
+template <class T>
+MAG_HD M3<T> mul_hn(const M3<T>& h, const M3<T>& a) {
+  M3<T> c;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = {T(0), T(0)};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s = (k == i) ? rfma(h(i, i).r, a(k, j), s) : cfma(h(i, k), a(k, j), s);
+      c(i, j) = s;
+    }
+  return c;
+}
+// C += A H
+template <class T>
+MAG_HD void mul_acc_nh(M3<T>& c, const M3<T>& a, const M3<T>& h) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = c(i, j);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s = (k == j) ? rfma(h(j, j).r, a(i, k), s) : cfma(a(i, k), h(k, j), s);
+      c(i, j) = s;
+    }
+}
+// C -= A H
+template <class T>
+MAG_HD void mul_sub_nh(M3<T>& c, const M3<T>& a, const M3<T>& h) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = c(i, j);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s = (k == j) ? rfma(-h(j, j).r, a(i, k), s) : cfma(neg(a(i, k)), h(k, j), s);
+      c(i, j) = s;
+    }
+}
+// C = A^+ H
+template <class T>
+MAG_HD M3<T> mul_dh(const M3<T>& a, const M3<T>& h) {
+  M3<T> c;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = {T(0), T(0)};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s = (k == j) ? rfma(h(j, j).r, cconj(a(k, i)), s) : cfma_cj(a(k, i), h(k, j), s);
+      c(i, j) = s;
+    }
+  return c;
+}
+// C -= H A^+
+template <class T>
+MAG_HD void mul_hd_sub(M3<T>& c, const M3<T>& h, const M3<T>& a) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = c(i, j);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        s = (k == i) ? rfma(-h(i, i).r, cconj(a(j, k)), s) : cfma(neg(h(i, k)), cconj(a(j, k)), s);
+      c(i, j) = s;
+    }
+}
+
 // C = A^+ B^+ = (B A)^+
 template <class T>
 MAG_HD M3<T> mul_dd(const M3<T>& a, const M3<T>& b) {
