@@ -33,6 +33,7 @@ typedef struct mag_ctx_s* mag_ctx;
 typedef struct mag_field_s* mag_field;
 typedef struct mag_chain_s* mag_chain;
 typedef struct mag_server_s* mag_server;
+typedef struct mag_multi_s* mag_multi;
 
 /* status codes */
 enum {
@@ -49,8 +50,10 @@ enum {
   MAG_ERR_CHECKSUM = -10    /* stored checksum or plaquette does not match the data */
 };
 
-/* field kinds and precisions */
-enum { MAG_LINK18 = 0, MAG_LINK12 = 1, MAG_ALG8 = 2 };
+/* field kinds and precisions; MAG_REAL1 holds one real per link (the xi
+ * field of the coarse/fine HMC, qlat Field<double> of multiplicity 4, host
+ * layout [site][mu]) and is FP64 only */
+enum { MAG_LINK18 = 0, MAG_LINK12 = 1, MAG_ALG8 = 2, MAG_REAL1 = 3 };
 enum { MAG_F32 = 0, MAG_F64 = 1 };
 /* gauge actions: Wilson, or the Iwasaki form with c0 = 1 - 8 c1 (Iwasaki
  * c1 = -0.331, DBW2 c1 = -1.4069); Gauge{type, c1}, field-hmc.h:344, 384 */
@@ -206,6 +209,73 @@ int mag_constrained_plaquette(mag_field U, int mag, double* out);
 /* lattice_expansion (expansion/main.C:19-62) without the file I/O: unit
  * links, the coarse link at fine site mag*x */
 int mag_lattice_expansion(mag_field coarse, int mag, mag_field fine);
+
+/* ------------------------------------------------ coupled coarse/fine HMC
+ * The md:: HMC of field-multi-md.h, in FP64. Fine links UF (MAG_LINK18, even
+ * extents) and coarse links UC of half the fine extents; coarse site c sits
+ * at fine site 2c. Per coarse link: U2 = U(2c) U(2c+mu), Q = -i rho
+ * TA(stp U2^+) with the six 2x1 staples stp (get_Q, field-multi.h:33-39),
+ * Gb = e^{iQ} U2, S_xi = -(1 + md_alpha) sum xi Re Tr(UC^+ Gb). xi is a
+ * MAG_REAL1 field of the coarse shape. The reference leaves rho and XI0 to
+ * its includer and fixes md_alpha = 0.5 (field-multi-md.h:28). */
+typedef struct {
+  double rho;       /* stout weight of get_Q */
+  double xi0;       /* XI0: the value init_xi writes, and the heat bath's weight */
+  double md_alpha;  /* md_alpha */
+} mag_multi_params;
+
+/* get_xi_energy (field-multi-md.h:30-72) */
+int mag_xi_energy(mag_field UF, mag_field UC, mag_field xi, const mag_multi_params* mp, double* out);
+/* get_Gb (field-multi-md.h:771-783): Gb = e^{iQ} U2 into a coarse link field */
+int mag_multi_gb(mag_field Gb, mag_field UF, const mag_multi_params* mp);
+/* md::get_Fforce (field-multi-md.h:116-368): thin gauge force (no link
+ * classes) plus the stout term of S_xi / (1 + md_alpha). F is fine MAG_ALG8;
+ * act must be thin (n_stout = 0). */
+int mag_multi_fforce(mag_field F, mag_field UF, mag_field UC, mag_field xi, const mag_action* act,
+                     const mag_multi_params* mp);
+/* md::get_Cforce (field-multi-md.h:398-425): coarse momentum
+ * -i md_alpha/(1+md_alpha) (dGb/dt) Gb^+ along the fine momenta PF; CF is
+ * coarse MAG_ALG8 */
+int mag_multi_cforce(mag_field CF, mag_field UF, mag_field PF, const mag_multi_params* mp);
+/* heatbath (field-multi-md.h:757-769): one simplest_metropolis hit per
+ * coarse link with weight (1 + md_alpha) XI0 Re Tr(UC Gb^+); Philox stream
+ * keyed by (seed, coarse link, traj) */
+int mag_multi_heatbath(mag_field UC, mag_field UF, const mag_multi_params* mp, uint64_t seed, uint64_t traj);
+/* force_gradient_integrator_zeta_T (field-multi-md.h:441-475); mag is
+ * evolve_gauge_field's constraint block size (Arg_chmc::mag) */
+int mag_multi_zeta_T(mag_field UF, mag_field PF, mag_field UC, double dt, int mag, const mag_multi_params* mp);
+/* integrators: force_gradient_integrator_zeta_H (477-597), the md::
+ * force_gradient_integrator with the coarse links fixed (599-661; dt is the
+ * reference's Farg.dt) and nested_integrator (663-695). Both fields are
+ * reunitarized at the end of each FG pass; max_dev = {fine, coarse}. */
+enum { MAG_MULTI_ZETA_H = 0, MAG_MULTI_FG = 1, MAG_MULTI_NESTED = 2 };
+int mag_multi_integrate(mag_field UF, mag_field PF, mag_field UC, mag_field xi, const mag_action* act,
+                        const mag_multi_params* mp, int which, int steps, double dt, double max_dev[2]);
+/* md::get_hamiltonian_multi (field-multi-md.h:74-114): parts = {kinetic
+ * (all links), thin gauge action, S_xi} */
+int mag_multi_hamiltonian(mag_field UF, mag_field PF, mag_field UC, mag_field xi, const mag_action* act,
+                          const mag_multi_params* mp, double parts[3], double* H);
+
+/* md::run_hmc_multi (field-multi-md.h:785-988) as a device-resident chain:
+ * per trajectory init_momentum, heatbath, init_xi, H, zeta_H, H and the
+ * Metropolis step on both fields (forced accepts first). A reject restores
+ * both fields to the last accepted state, dropping that trajectory's heat-bath
+ * update as the reference does. Uses trajectory_length, dt,
+ * num_forced_accept and seed of mag_hmc_params; precision must be MAG_F64.
+ * Both fields start as unit links; set them through mag_multi_fields. */
+typedef struct {
+  double h_old, h_new, dh, die_roll;
+  int accept;
+  double avg_plaq_fine, avg_plaq_coarse;  /* of the state after the accept step */
+  double parts_old[3], parts_new[3];      /* kinetic, gauge, xi */
+  double reunit_dev[2];                   /* fine, coarse */
+  int status;                             /* MAG_OK or MAG_ERR_NAN */
+} mag_multi_result;
+int mag_multi_create(mag_ctx ctx, const int fine_dims[4], const mag_action* act, const mag_hmc_params* p,
+                     const mag_multi_params* mp, mag_multi* out);
+int mag_multi_destroy(mag_multi m);
+int mag_multi_fields(mag_multi m, mag_field* UF, mag_field* UC); /* borrowed */
+int mag_multi_run(mag_multi m, int ntraj, mag_multi_result* results);
 
 /* ---------------------------------------------------------- NERSC config I/O
  * export_config_nersc / import_config_nersc (field-matrix.h:445-713): text

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field as dfield
 import numpy as np
 
 from . import _lib
-from ._lib import (ALG8, F32, F64, IWASAKI, LINK12, LINK18, WILSON, Action, HmcParams, MagError,
+from ._lib import (ALG8, F32, F64, IWASAKI, LINK12, LINK18, REAL1, WILSON, Action, HmcParams, MagError,
                    TrajResult)
 
 __all__ = [
@@ -127,13 +127,17 @@ class Field:
         except Exception:
             pass
 
+    def _host_shape(self):
+        # REAL1: one real per link ([site][mu]); otherwise 3x3 complex per link
+        return (self.volume, 4) if self.kind == REAL1 else (self.volume, 4, 18)
+
     def upload(self, host):
-        a = np.ascontiguousarray(host, dtype=np.float64).reshape(self.volume, 4, 18)
+        a = np.ascontiguousarray(host, dtype=np.float64).reshape(self._host_shape())
         _check(_lib.lib().mag_field_upload_aos(self._h, _dptr(a)), self.ctx.handle)
         return self
 
     def download(self):
-        a = np.empty((self.volume, 4, 18))
+        a = np.empty(self._host_shape())
         _check(_lib.lib().mag_field_download_aos(self._h, _dptr(a)), self.ctx.handle)
         return a
 

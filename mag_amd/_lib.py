@@ -17,7 +17,7 @@ STATUS_NAMES = {0: "MAG_OK", -1: "MAG_ERR_ARG", -2: "MAG_ERR_GEOMETRY", -3: "MAG
                 -5: "MAG_ERR_NAN", -6: "MAG_ERR_UNSUPPORTED", -7: "MAG_ERR_BUDGET", -8: "MAG_ERR_STATE",
                 -9: "MAG_ERR_IO", -10: "MAG_ERR_CHECKSUM"}
 # field kinds / precisions / actions
-LINK18, LINK12, ALG8 = 0, 1, 2
+LINK18, LINK12, ALG8, REAL1 = 0, 1, 2, 3
 F32, F64 = 0, 1
 WILSON, IWASAKI = 0, 1
 
@@ -48,6 +48,17 @@ class NerscHeader(C.Structure):
 class ServeRequest(C.Structure):
     _fields_ = [("dims", C.c_int * 4), ("act", Action), ("hmc", HmcParams), ("ntraj", C.c_int),
                 ("links_seed", C.c_uint64)]
+
+
+class MultiParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("xi0", C.c_double), ("md_alpha", C.c_double)]
+
+
+class MultiResult(C.Structure):
+    _fields_ = [("h_old", C.c_double), ("h_new", C.c_double), ("dh", C.c_double), ("die_roll", C.c_double),
+                ("accept", C.c_int), ("avg_plaq_fine", C.c_double), ("avg_plaq_coarse", C.c_double),
+                ("parts_old", C.c_double * 3), ("parts_new", C.c_double * 3), ("reunit_dev", C.c_double * 2),
+                ("status", C.c_int)]
 
 
 _P = C.c_void_p
@@ -108,6 +119,19 @@ SIGNATURES = {
     "mag_update_constrain": [_P, _P, C.c_int],
     "mag_constrained_plaquette": [_P, C.c_int, _D],
     "mag_lattice_expansion": [_P, C.c_int, _P],
+    "mag_xi_energy": [_P, _P, _P, C.POINTER(MultiParams), _D],
+    "mag_multi_gb": [_P, _P, C.POINTER(MultiParams)],
+    "mag_multi_fforce": [_P, _P, _P, _P, C.POINTER(Action), C.POINTER(MultiParams)],
+    "mag_multi_cforce": [_P, _P, _P, C.POINTER(MultiParams)],
+    "mag_multi_heatbath": [_P, _P, C.POINTER(MultiParams), C.c_uint64, C.c_uint64],
+    "mag_multi_zeta_T": [_P, _P, _P, C.c_double, C.c_int, C.POINTER(MultiParams)],
+    "mag_multi_integrate": [_P, _P, _P, _P, C.POINTER(Action), C.POINTER(MultiParams), C.c_int, C.c_int,
+                            C.c_double, _D],
+    "mag_multi_hamiltonian": [_P, _P, _P, _P, C.POINTER(Action), C.POINTER(MultiParams), _D, _D],
+    "mag_multi_create": [_P, _I, C.POINTER(Action), C.POINTER(HmcParams), C.POINTER(MultiParams), C.POINTER(_P)],
+    "mag_multi_destroy": [_P],
+    "mag_multi_fields": [_P, C.POINTER(_P), C.POINTER(_P)],
+    "mag_multi_run": [_P, C.c_int, C.POINTER(MultiResult)],
     "mag_nersc_write": [C.c_char_p, _D, C.POINTER(NerscHeader)],
     "mag_nersc_read": [C.c_char_p, _D, C.POINTER(NerscHeader)],
     "mag_export_config_nersc": [_P, C.c_char_p, C.c_double, C.c_int, C.c_char_p, C.c_int],

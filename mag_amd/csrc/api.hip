@@ -31,12 +31,17 @@ int check_dims(mag_ctx c, const int* d) {
   return MAG_OK;
 }
 
-bool is_link(mag_field f) { return f && f->kind != MAG_ALG8; }
+bool is_link(mag_field f) { return f && (f->kind == MAG_LINK18 || f->kind == MAG_LINK12); }
 bool is_alg(mag_field f) { return f && f->kind == MAG_ALG8; }
 
 template <class TS>
 void aos_to_soa(const double* h, TS* s, const Geom& g, int kind) {
   const int nc = ncomp(kind);
+  if (kind == MAG_REAL1) {
+    for (int site = 0; site < g.V; ++site)
+      for (int mu = 0; mu < 4; ++mu) s[(size_t)mu * g.V + site] = TS(h[(size_t)site * 4 + mu]);
+    return;
+  }
   for (int site = 0; site < g.V; ++site)
     for (int mu = 0; mu < 4; ++mu) {
       const double* m = h + ((size_t)site * 4 + mu) * 18;
@@ -55,6 +60,11 @@ void aos_to_soa(const double* h, TS* s, const Geom& g, int kind) {
 template <class TS>
 void soa_to_aos(const TS* s, double* h, const Geom& g, int kind) {
   const int nc = ncomp(kind);
+  if (kind == MAG_REAL1) {
+    for (int site = 0; site < g.V; ++site)
+      for (int mu = 0; mu < 4; ++mu) h[(size_t)site * 4 + mu] = double(s[(size_t)mu * g.V + site]);
+    return;
+  }
   for (int site = 0; site < g.V; ++site)
     for (int mu = 0; mu < 4; ++mu) {
       double* m = h + ((size_t)site * 4 + mu) * 18;
@@ -190,8 +200,9 @@ int mag_field_create(mag_ctx c, const int dims[4], int kind, int prec, mag_field
   *out = nullptr;
   int r;
   if ((r = check_dims(c, dims))) return r;
-  if (kind < MAG_LINK18 || kind > MAG_ALG8 || (prec != MAG_F32 && prec != MAG_F64))
+  if (kind < MAG_LINK18 || kind > MAG_REAL1 || (prec != MAG_F32 && prec != MAG_F64))
     return fail(c, MAG_ERR_ARG, "bad field kind or precision");
+  if (kind == MAG_REAL1 && prec != MAG_F64) return fail(c, MAG_ERR_UNSUPPORTED, "real-per-link fields are FP64");
   if ((kind == MAG_LINK12 && prec == MAG_F64) || (kind == MAG_LINK18 && prec == MAG_F32))
     return fail(c, MAG_ERR_UNSUPPORTED, "link fields are FP64 18-real or FP32 12-real");
   auto* f = new mag_field_s();
@@ -263,7 +274,7 @@ int mag_field_copy(mag_field dst, mag_field src) {
     HIPCHK(c, hipMemcpyAsync(dst->p, src->p, dst->bytes, hipMemcpyDeviceToDevice, c->stream));
     return MAG_OK;
   }
-  if (!is_link(dst) || !is_link(src)) return fail(c, MAG_ERR_ARG, "cannot convert algebra <-> link fields");
+  if (!is_link(dst) || !is_link(src)) return fail(c, MAG_ERR_ARG, "only link fields convert between formats");
   return convert_links(c, dst->g, src->prec, src->kind, src->p, dst->prec, dst->kind, dst->p);
 }
 

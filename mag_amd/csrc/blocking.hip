@@ -130,7 +130,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(LinkView<TS, NC> uc, Geom gc, 
   else uf.store(mu, s, unit3<TS>());
 }
 
-bool linkf(mag_field f) { return f && f->kind != MAG_ALG8; }
+bool linkf(mag_field f) { return f && (f->kind == MAG_LINK18 || f->kind == MAG_LINK12); }
 
 // fine = mag x coarse in every direction, same format and context
 int check_pair(mag_field fine, mag_field coarse, int mag) {

@@ -194,7 +194,7 @@ int mag_export_config_nersc(mag_field U, const char* path, double beta, int sequ
   size_t bytes = 0;
   int r = mag_field_info(U, h.dims, &kind, &prec, &bytes);
   if (r) return r;
-  if (kind == MAG_ALG8) return fail(U->ctx, MAG_ERR_ARG, "export_config_nersc needs a link field");
+  if (kind != MAG_LINK18 && kind != MAG_LINK12) return fail(U->ctx, MAG_ERR_ARG, "export_config_nersc needs a link field");
   std::vector<double> aos((size_t)4 * volume(h.dims) * 18);
   if ((r = mag_field_download_aos(U, aos.data())) || (r = mag_avg_plaquette(U, &h.plaquette))) return r;
   double tr = 0.;  // site order, mu order (field-matrix.h:391-405)
@@ -215,7 +215,7 @@ int mag_import_config_nersc(mag_field U, const char* path, mag_nersc_header* hou
   size_t bytes = 0;
   int r = mag_field_info(U, dims, &kind, &prec, &bytes);
   if (r) return r;
-  if (kind == MAG_ALG8) return fail(U->ctx, MAG_ERR_ARG, "import_config_nersc needs a link field");
+  if (kind != MAG_LINK18 && kind != MAG_LINK12) return fail(U->ctx, MAG_ERR_ARG, "import_config_nersc needs a link field");
   mag_nersc_header h{};
   if ((r = mag_nersc_read(path, nullptr, &h))) return fail(U->ctx, r, std::string("cannot read header of ") + path);
   for (int i = 0; i < 4; ++i)

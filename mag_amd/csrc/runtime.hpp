@@ -104,7 +104,9 @@ inline int fail(mag_ctx c, int code, const std::string& msg) {
 
 #define CHECK_LAUNCH(ctx) HIPCHK(ctx, hipGetLastError())
 
-inline int ncomp(int kind) { return kind == MAG_LINK18 ? 18 : (kind == MAG_LINK12 ? 12 : 8); }
+inline int ncomp(int kind) {
+  return kind == MAG_LINK18 ? 18 : (kind == MAG_LINK12 ? 12 : (kind == MAG_ALG8 ? 8 : 1));
+}
 inline size_t real_size(int prec) { return prec == MAG_F64 ? 8 : 4; }
 
 inline hipEvent_t take_event(mag_ctx c) {
@@ -243,6 +245,8 @@ struct Workspace {
   void* L[2] = {nullptr, nullptr};  // stout Lambda (alg8), ping-pong between levels
   void* W = nullptr;
   void* Uh[2] = {nullptr, nullptr};  // FP64 smeared fields for the Hamiltonian
+  void* Lam = nullptr;  // alg8: Lambda of the coarse/fine HMC (coarse shape)
+  void* Eq = nullptr;   // mat18: e^{iQ} of the coarse/fine HMC (coarse shape)
 
   ~Workspace() {
     for (auto& a : allocs) dev_free(c, a.first, a.second);
@@ -279,6 +283,11 @@ struct Workspace {
     return get(&W, mat_bytes());
   }
   int need_Z() { return get(&Z, mat_bytes()); }
+  int need_multi() {
+    int r = get(&Lam, alg_bytes());
+    if (r) return r;
+    return get(&Eq, mat_bytes());
+  }
   int need_h(int n) {
     int r = get(&Uh[0], mat64_bytes());
     if (r || n < 2) return r;

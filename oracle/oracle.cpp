@@ -778,6 +778,412 @@ double avg_wilson_loop(const Field& f, const int* r, int T) {
   return s / (perm.size() * (double)f.g->V);
 }
 
+// ---------------------------------------------------------------- md:: coarse/fine HMC
+// field-multi.h and field-multi-md.h. The coarse lattice has half the fine
+// extents; coarse site xc sits at fine site x = 2 xc. rho, XI0 and md_alpha
+// are parameters (the reference leaves rho and XI0 to the includer).
+enum : uint32_t { ST_HEATBATH = 5 };
+
+// field-multi.h:27-31
+Mat get_U2(const Field& f, const int* x, int mu) {
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  y[mu]++;
+  return f.at(x, mu) * f.at(y, mu);
+}
+
+// field-hmc.h:244-260: the six 2x1 staples from x to x + 2 mu
+Mat staple_rect(const Field& f, const int* x, int mu) {
+  Mat s = Mat::zero();
+  for (int nu = 0; nu < DIMN; ++nu) {
+    if (nu == mu) continue;
+    int d1[4] = {nu, mu, mu, nu + DIMN};
+    s += path_product(f, x, d1, 4);
+    int d2[4] = {nu + DIMN, mu, mu, nu};
+    s += path_product(f, x, d2, 4);
+  }
+  return s;
+}
+
+// field-multi.h:33-39: Q = -i rho TA(stp U2^+)
+Mat get_Q(const Field& f, const int* x, int mu, double rho) {
+  Mat Omega = staple_rect(f, x, mu) * dagger(get_U2(f, x, mu));
+  return trless_antiherm(Omega) * cplx(0., -rho);
+}
+
+// field-matrix.h:177-200; a direction of -1 multiplies by ins
+Mat path_insertion(const Field& f, const int* x, const int* dir, int n, const Mat& ins) {
+  Mat mul = Mat::unit();
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  for (int i = 0; i < n; ++i) {
+    int d = dir[i];
+    if (d == -1) {
+      mul = mul * ins;
+    } else if (d < DIMN) {
+      mul = mul * f.at(y, d);
+      y[d]++;
+    } else {
+      y[d - DIMN]--;
+      mul = mul * dagger(f.at(y, d - DIMN));
+    }
+  }
+  return mul;
+}
+
+// field-matrix.h:138-175: d/dt of the path product with U' = i P U on
+// every link, summed over the positions of the path
+Mat path_leftD(const Field& g, const Field& m, const int* x, const int* dir, int n) {
+  Mat add = Mat::zero();
+  for (int d = 0; d < n; ++d) {
+    int y[4] = {x[0], x[1], x[2], x[3]};
+    Mat mul = Mat::unit();
+    for (int i = 0; i < n; ++i) {
+      int k = dir[i];
+      if (k < DIMN) {
+        if (i == d) mul = mul * m.at(y, k) * cplx(0., 1.) * g.at(y, k);
+        else mul = mul * g.at(y, k);
+        y[k]++;
+      } else {
+        y[k - DIMN]--;
+        if (i == d) mul = mul * dagger(m.at(y, k - DIMN) * cplx(0., 1.) * g.at(y, k - DIMN));
+        else mul = mul * dagger(g.at(y, k - DIMN));
+      }
+    }
+    add += mul;
+  }
+  return add;
+}
+
+// field-multi.h:384-407: {type, nu} of fine link (x, mu)
+void stout_type(const int* x, int mu, int out[2]) {
+  int cnt[4], n = 0;
+  for (int i = 0; i < DIMN; ++i)
+    if (x[i] % 2 == 1) cnt[n++] = i;
+  out[0] = 0; out[1] = 0;
+  if (n == 0) { out[0] = 1; return; }
+  if (n == 1) {
+    if (cnt[0] == mu) out[0] = 3;
+    else { out[0] = 2; out[1] = cnt[0]; }
+    return;
+  }
+  if (n == 2) {
+    if (cnt[0] == mu) { out[0] = 4; out[1] = cnt[1]; return; }
+    if (cnt[1] == mu) { out[0] = 4; out[1] = cnt[0]; return; }
+  }
+}
+
+struct Multi {
+  const Geo& gf;
+  const Geo& gc;
+  double rho, xi0, alpha;
+};
+
+void half(const int* y, int* yc) { for (int i = 0; i < 4; ++i) yc[i] = y[i] / 2; }
+
+// md::get_Fforce (field-multi-md.h:116-368): the thin gauge force without
+// link classes, plus the stout term of every fine link by its stout_type.
+void multi_fforce(const Multi& M, const std::vector<Mat>& UF, const std::vector<Mat>& UC,
+                  const std::vector<double>& xi, const orc_action& a, std::vector<Mat>& F) {
+  orc_action thin = a;
+  thin.mag = 0;
+  thin.n_stout = 0;
+  get_force_thin(M.gf, UF, thin, F);
+  Field f{&M.gf, const_cast<Mat*>(UF.data())}, c{&M.gc, const_cast<Mat*>(UC.data())};
+  std::vector<Mat> Lam(M.gc.V * 4);
+  // 164-176
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < M.gc.V; ++s) {
+    int xc[4], x[4];
+    M.gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    for (int mu = 0; mu < DIMN; ++mu)
+      Lam[s * 4 + mu] = compute_Lambda(get_Q(f, x, mu, M.rho), dagger(c.at(s, mu)), get_U2(f, x, mu));
+  }
+  Field cl{&M.gc, Lam.data()};
+  const double rho = M.rho;
+  auto LAM = [&](const int* y, int mu) { int yc[4]; half(y, yc); return cl.at(yc, mu); };
+  auto XI = [&](const int* y, int mu) { int yc[4]; half(y, yc); return xi[M.gc.index(yc) * 4 + mu]; };
+  auto CG = [&](const int* y, int mu) { int yc[4]; half(y, yc); return c.at(yc, mu); };
+  const int D = DIMN;
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < M.gf.V; ++s) {
+    int x[4];
+    M.gf.coord(s, x);
+    for (int mu = 0; mu < DIMN; ++mu) {
+      Mat t = Mat::zero();
+      int sx[4] = {x[0], x[1], x[2], x[3]};
+      sx[mu]++;
+      int tn[2];
+      stout_type(x, mu, tn);
+      int y[4];
+      auto set = [&](int d0, int n0, int d1, int n1) {
+        for (int i = 0; i < 4; ++i) y[i] = x[i];
+        y[d0] += n0;
+        y[d1] += n1;
+      };
+      switch (tn[0]) {
+        case 1: {  // 196-252
+          set(mu, 1, mu, 0);
+          t += f.at(y, mu) * dagger(CG(x, mu)) * expiQ(get_Q(f, x, mu, rho)) * XI(x, mu);
+          for (int nu = 0; nu < DIMN; ++nu) {
+            if (nu == mu) continue;
+            int p1[6] = {mu, nu, mu + D, mu + D, nu + D, -1};
+            t += path_insertion(f, sx, p1, 6, LAM(x, mu)) * cplx(0., rho) * XI(x, mu);
+            int p2[6] = {mu, nu + D, mu + D, mu + D, nu, -1};
+            t += path_insertion(f, sx, p2, 6, LAM(x, mu)) * cplx(0., rho) * XI(x, mu);
+            int p3[6] = {nu, nu, mu + D, nu + D, nu + D, -1};
+            t += path_insertion(f, sx, p3, 6, LAM(x, nu)) * cplx(0., -rho) * XI(x, nu);
+            set(nu, -2, nu, 0);
+            int p4[6] = {nu + D, nu + D, mu + D, -1, nu, nu};
+            t += path_insertion(f, sx, p4, 6, LAM(y, nu)) * cplx(0., rho) * XI(y, nu);
+          }
+          break;
+        }
+        case 2: {  // 254-278
+          const int nu = tn[1];
+          set(nu, -1, nu, 0);
+          int p1[6] = {mu, nu + D, mu + D, mu + D, -1, nu};
+          t += path_insertion(f, sx, p1, 6, LAM(y, mu)) * cplx(0., -rho) * XI(y, mu);
+          set(nu, 1, nu, 0);
+          int p2[6] = {mu, nu, mu + D, mu + D, -1, nu + D};
+          t += path_insertion(f, sx, p2, 6, LAM(y, mu)) * cplx(0., -rho) * XI(y, mu);
+          break;
+        }
+        case 3: {  // 279-332
+          set(mu, -1, mu, 0);
+          t += dagger(CG(y, mu)) * expiQ(get_Q(f, y, mu, rho)) * f.at(y, mu) * XI(y, mu);
+          for (int nu = 0; nu < DIMN; ++nu) {
+            if (nu == mu) continue;
+            set(mu, -1, mu, 0);
+            int p1[6] = {nu, mu + D, mu + D, nu + D, -1, mu};
+            t += path_insertion(f, sx, p1, 6, LAM(y, mu)) * cplx(0., rho) * XI(y, mu);
+            int p2[6] = {nu + D, mu + D, mu + D, nu, -1, mu};
+            t += path_insertion(f, sx, p2, 6, LAM(y, mu)) * cplx(0., rho) * XI(y, mu);
+            set(mu, 1, mu, 0);
+            int p3[6] = {-1, nu, nu, mu + D, nu + D, nu + D};
+            t += path_insertion(f, sx, p3, 6, LAM(y, nu)) * cplx(0., rho) * XI(y, nu);
+            set(mu, 1, nu, -2);
+            int p4[6] = {nu + D, nu + D, -1, mu + D, nu, nu};
+            t += path_insertion(f, sx, p4, 6, LAM(y, nu)) * cplx(0., -rho) * XI(y, nu);
+          }
+          break;
+        }
+        case 4: {  // 333-357
+          const int nu = tn[1];
+          set(nu, -1, mu, -1);
+          int p1[6] = {nu + D, mu + D, mu + D, -1, nu, mu};
+          t += path_insertion(f, sx, p1, 6, LAM(y, mu)) * cplx(0., -rho) * XI(y, mu);
+          set(nu, 1, mu, -1);
+          int p2[6] = {nu, mu + D, mu + D, -1, nu + D, mu};
+          t += path_insertion(f, sx, p2, 6, LAM(y, mu)) * cplx(0., -rho) * XI(y, mu);
+          break;
+        }
+        default: break;
+      }
+      // 363-365
+      Mat w = trless_antiherm(f.at(s, mu) * t);
+      F[s * 4 + mu] += w * cplx(0., 1.);
+    }
+  }
+}
+
+// get_DQ (field-multi-md.h:370-396)
+Mat get_DQ(const Field& g, const Field& p, const int* x, int mu, double rho) {
+  Mat series = Mat::zero();
+  for (int nu = 0; nu < DIMN; ++nu) {
+    if (nu == mu) continue;
+    int d1[6] = {nu, mu, mu, nu + DIMN, mu + DIMN, mu + DIMN};
+    series += path_leftD(g, p, x, d1, 6);
+    int d2[6] = {nu + DIMN, mu, mu, nu, mu + DIMN, mu + DIMN};
+    series += path_leftD(g, p, x, d2, 6);
+  }
+  return trless_antiherm(series) * (I() * -rho);
+}
+
+// get_Cforce (field-multi-md.h:398-425): the coarse links' momentum, the
+// velocity of Gb = e^{iQ} U2 along the fine momenta, scaled by alpha/(1+alpha)
+void multi_cforce(const Multi& M, const std::vector<Mat>& UF, const std::vector<Mat>& PF, std::vector<Mat>& CF) {
+  CF.resize(M.gc.V * 4);
+  Field g{&M.gf, const_cast<Mat*>(UF.data())}, p{&M.gf, const_cast<Mat*>(PF.data())};
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < M.gc.V; ++s) {
+    int xc[4], x[4];
+    M.gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    for (int mu = 0; mu < DIMN; ++mu) {
+      int dir[2] = {mu, mu};
+      Mat Q = get_Q(g, x, mu, M.rho);
+      Mat U = get_U2(g, x, mu);
+      Mat DQ = get_DQ(g, p, x, mu, M.rho);
+      Mat DU = path_leftD(g, p, x, dir, 2);
+      Mat t = (DexpiQ(Q, DQ) * U + expiQ(Q) * DU) * dagger(expiQ(Q) * U);
+      CF[s * 4 + mu] = t * (-1. * I() * M.alpha / (1. + M.alpha));
+    }
+  }
+}
+
+// get_xi_energy (field-multi-md.h:30-72)
+double xi_energy(const Multi& M, const std::vector<Mat>& UF, const std::vector<Mat>& UC,
+                 const std::vector<double>& xi) {
+  Field f{&M.gf, const_cast<Mat*>(UF.data())}, c{&M.gc, const_cast<Mat*>(UC.data())};
+  std::vector<double> v(M.gc.V);
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < M.gc.V; ++s) {
+    int xc[4], x[4];
+    M.gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    double a = 0.;
+    for (int mu = 0; mu < DIMN; ++mu) {
+      Mat Gb = expiQ(get_Q(f, x, mu, M.rho)) * get_U2(f, x, mu);
+      a += retr(dagger(c.at(s, mu)) * Gb) * xi[s * 4 + mu];
+    }
+    v[s] = a;
+  }
+  double sum = 0.;
+  for (double a : v) sum += a;
+  return -sum * (1. + M.alpha);
+}
+
+// get_Gb (field-multi-md.h:771-783)
+void multi_gb(const Multi& M, const std::vector<Mat>& UF, std::vector<Mat>& Gb) {
+  Gb.resize(M.gc.V * 4);
+  Field f{&M.gf, const_cast<Mat*>(UF.data())};
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < M.gc.V; ++s) {
+    int xc[4], x[4];
+    M.gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    for (int mu = 0; mu < DIMN; ++mu) Gb[s * 4 + mu] = expiQ(get_Q(f, x, mu, M.rho)) * get_U2(f, x, mu);
+  }
+}
+
+// heatbath + simplest_metropolis (field-multi-md.h:748-769) with the
+// Philox heat-bath stream keyed by (coarse link, trajectory): draws 0-1 give
+// a0..a3 of random_su3_from_su2(0.9) (field-matrix.h:36-65), draw 2 the
+// SU(2) subgroup and the Metropolis uniform.
+void heatbath(const Multi& M, std::vector<Mat>& UC, const std::vector<Mat>& UF, uint64_t seed, uint64_t traj) {
+  Field f{&M.gf, const_cast<Mat*>(UF.data())};
+  const double coeff = (1. + M.alpha) * M.xi0;
+#pragma omp parallel for schedule(static)
+  for (long s = 0; s < M.gc.V; ++s) {
+    int xc[4], x[4];
+    M.gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    for (int mu = 0; mu < DIMN; ++mu) {
+      Mat env = expiQ(get_Q(f, x, mu, M.rho)) * get_U2(f, x, mu);
+      uint32_t w0[4], w1[4], w2[4];
+      const uint32_t link = (uint32_t)(s * 4 + mu);
+      draw(seed, ST_HEATBATH, traj, link, 0, w0);
+      draw(seed, ST_HEATBATH, traj, link, 1, w1);
+      draw(seed, ST_HEATBATH, traj, link, 2, w2);
+      double a0 = 0.9 + (1. - 0.9) * u01(w0[0], w0[1]);
+      double a1 = -1. + 2. * u01(w0[2], w0[3]);
+      double a2 = -1. + 2. * u01(w1[0], w1[1]);
+      double a3 = -1. + 2. * u01(w1[2], w1[3]);
+      double nrm = std::sqrt((1. - a0 * a0) / (a1 * a1 + a2 * a2 + a3 * a3));
+      a1 *= nrm; a2 *= nrm; a3 *= nrm;
+      Mat m = Mat::unit();
+      switch (w2[0] % 3) {
+        case 0: m(1, 1) = cplx(a0, a3); m(1, 2) = cplx(a2, a1); m(2, 1) = cplx(-a2, a1); m(2, 2) = cplx(a0, -a3); break;
+        case 1: m(0, 0) = cplx(a0, a3); m(0, 2) = cplx(a2, a1); m(2, 0) = cplx(-a2, a1); m(2, 2) = cplx(a0, -a3); break;
+        case 2: m(0, 0) = cplx(a0, a3); m(0, 1) = cplx(a2, a1); m(1, 0) = cplx(-a2, a1); m(1, 1) = cplx(a0, -a3); break;
+      }
+      Mat& var = UC[s * 4 + mu];
+      Mat nv = var * m;
+      double diff = coeff * retr(nv * dagger(env) - var * dagger(env));
+      if (u01(w2[2], w2[3]) < std::exp(diff)) var = nv;
+    }
+  }
+}
+
+// force_gradient_integrator_zeta_T (field-multi-md.h:441-475)
+void zeta_T(const Multi& M, std::vector<Mat>& UF, const std::vector<Mat>& PF, std::vector<Mat>& UC,
+            std::vector<Mat>& Cm, double dt, int mag) {
+  const double alpha = 1. / 6., beta = 2. / 3.;
+  evolve_gauge_field(M.gf, UF, PF, alpha * dt, mag);
+  multi_cforce(M, UF, PF, Cm);
+  evolve_gauge_field(M.gc, UC, Cm, 0.5 * dt, mag);
+  evolve_gauge_field(M.gf, UF, PF, beta * dt, mag);
+  multi_cforce(M, UF, PF, Cm);
+  evolve_gauge_field(M.gc, UC, Cm, 0.5 * dt, mag);
+  evolve_gauge_field(M.gf, UF, PF, alpha * dt, mag);
+}
+
+// force_gradient_integrator_zeta_H (field-multi-md.h:477-597)
+void zeta_H(const Multi& M, std::vector<Mat>& UF, std::vector<Mat>& PF, std::vector<Mat>& UC,
+            const std::vector<double>& xi, const orc_action& a, int steps, double dt, double dev[2]) {
+  const double alpha = (3. - std::sqrt(3.)) * dt / 6.;
+  const double beta = dt / std::sqrt(3.);
+  const double gamma = (2. - std::sqrt(3.)) * dt * dt / 12.;
+  std::vector<Mat> Ff, FgA, CgA, Cm;
+  zeta_T(M, UF, PF, UC, Cm, alpha, a.mag);
+  for (int i = 0; i < steps; ++i) {
+    for (int h = 0; h < 2; ++h) {
+      multi_fforce(M, UF, UC, xi, a, Ff);
+      FgA = UF;
+      CgA = UC;
+      zeta_T(M, FgA, Ff, CgA, Cm, gamma, a.mag);
+      multi_fforce(M, FgA, CgA, xi, a, Ff);
+      evolve_momentum(PF, Ff, 0.5 * dt, 1.);
+      if (h == 0) zeta_T(M, UF, PF, UC, Cm, beta, a.mag);
+    }
+    zeta_T(M, UF, PF, UC, Cm, i < steps - 1 ? 2. * alpha : alpha, a.mag);
+  }
+  dev[0] = reunitarize(UF);
+  dev[1] = reunitarize(UC);
+}
+
+// md::force_gradient_integrator (field-multi-md.h:599-661): fine fields only,
+// the coarse links fixed. Reads its step size from dt (the reference's Farg.dt).
+void multi_fg(const Multi& M, std::vector<Mat>& UF, std::vector<Mat>& PF, std::vector<Mat>& UC,
+              const std::vector<double>& xi, const orc_action& a, int steps, double dt, double dev[2]) {
+  const double alpha = (3. - std::sqrt(3.)) * dt / 6.;
+  const double beta = dt / std::sqrt(3.);
+  const double gamma = (2. - std::sqrt(3.)) * dt * dt / 12.;
+  std::vector<Mat> Ff, FgA;
+  evolve_gauge_field(M.gf, UF, PF, alpha, a.mag);
+  for (int i = 0; i < steps; ++i) {
+    for (int h = 0; h < 2; ++h) {
+      multi_fforce(M, UF, UC, xi, a, Ff);
+      FgA = UF;
+      evolve_gauge_field(M.gf, FgA, Ff, gamma, a.mag);
+      multi_fforce(M, FgA, UC, xi, a, Ff);
+      evolve_momentum(PF, Ff, 0.5 * dt, 1.);
+      if (h == 0) evolve_gauge_field(M.gf, UF, PF, beta, a.mag);
+    }
+    evolve_gauge_field(M.gf, UF, PF, i < steps - 1 ? 2. * alpha : alpha, a.mag);
+  }
+  dev[0] = reunitarize(UF);
+  dev[1] = reunitarize(UC);
+}
+
+// nested_integrator (field-multi-md.h:663-695)
+void multi_nested(const Multi& M, std::vector<Mat>& UF, std::vector<Mat>& PF, std::vector<Mat>& UC,
+                  const std::vector<double>& xi, const orc_action& a, int steps, double dt, double dev[2]) {
+  std::vector<Mat> Cm;
+  multi_cforce(M, UF, PF, Cm);
+  evolve_gauge_field(M.gc, UC, Cm, dt / 2., a.mag);
+  for (int i = 0; i < steps; ++i) {
+    multi_fg(M, UF, PF, UC, xi, a, 1, dt, dev);
+    multi_cforce(M, UF, PF, Cm);
+    evolve_gauge_field(M.gc, UC, Cm, i < steps - 1 ? dt : dt / 2., a.mag);
+  }
+}
+
+// md::get_hamiltonian_multi (field-multi-md.h:74-114): parts = {kinetic
+// (all links, field-multi.h:271-301), thin gauge action, S_xi}
+double multi_hamiltonian(const Multi& M, const std::vector<Mat>& UF, const std::vector<Mat>& PF,
+                         const std::vector<Mat>& UC, const std::vector<double>& xi, const orc_action& a,
+                         double* parts) {
+  Field p{&M.gf, const_cast<Mat*>(PF.data())};
+  orc_action thin = a;
+  thin.n_stout = 0;
+  const double K = kinetic(p, 0);
+  const double S = gauge_action_of(M.gf, UF, thin);
+  const double X = xi_energy(M, UF, UC, xi);
+  if (parts) { parts[0] = K; parts[1] = S; parts[2] = X; }
+  return K + S + X;
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -1169,6 +1575,166 @@ void orc_expand(const int* Lc, const double* Uc, int mag, double* Uf) {
     for (int mu = 0; mu < 4; ++mu) f.at(x, mu) = c[s * 4 + mu];
   }
   store_field(u, Uf);
+}
+
+// ------------------------------------------------- md:: coarse/fine HMC
+// Fine extents Lf; the coarse field has extents Lf/2. xi: [coarse site][mu].
+static Multi make_multi(const Geo& gf, const Geo& gc, const orc_multi* mp) {
+  return Multi{gf, gc, mp->rho, mp->xi0, mp->md_alpha};
+}
+static void coarse_dims(const int* Lf, int* Lc) { for (int i = 0; i < 4; ++i) Lc[i] = Lf[i] / 2; }
+
+#define MULTI_GEO                \
+  Geo gf(Lf);                    \
+  int Lc_[4];                    \
+  coarse_dims(Lf, Lc_);          \
+  Geo gc(Lc_);                   \
+  Multi M = make_multi(gf, gc, mp)
+
+static std::vector<double> load_xi(const Geo& gc, const double* xi) { return std::vector<double>(xi, xi + gc.V * 4); }
+
+double orc_xi_energy(const int* Lf, const double* UF, const double* UC, const double* xi, const orc_multi* mp) {
+  MULTI_GEO;
+  return xi_energy(M, load_field(gf, UF), load_field(gc, UC), load_xi(gc, xi));
+}
+
+void orc_multi_gb(const int* Lf, const double* UF, const orc_multi* mp, double* Gb) {
+  MULTI_GEO;
+  std::vector<Mat> g;
+  multi_gb(M, load_field(gf, UF), g);
+  store_field(g, Gb);
+}
+
+void orc_multi_Q(const int* Lf, const double* UF, const orc_multi* mp, double* Q) {
+  MULTI_GEO;
+  std::vector<Mat> u = load_field(gf, UF), q(gc.V * 4);
+  Field f{&gf, u.data()};
+  for (long s = 0; s < gc.V; ++s) {
+    int xc[4], x[4];
+    gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    for (int mu = 0; mu < 4; ++mu) q[s * 4 + mu] = get_Q(f, x, mu, M.rho);
+  }
+  store_field(q, Q);
+}
+
+void orc_multi_fforce(const int* Lf, const double* UF, const double* UC, const double* xi, const orc_action* a,
+                      const orc_multi* mp, double* F) {
+  MULTI_GEO;
+  std::vector<Mat> f;
+  multi_fforce(M, load_field(gf, UF), load_field(gc, UC), load_xi(gc, xi), *a, f);
+  store_field(f, F);
+}
+
+void orc_multi_cforce(const int* Lf, const double* UF, const double* PF, const orc_multi* mp, double* CF) {
+  MULTI_GEO;
+  std::vector<Mat> cf;
+  multi_cforce(M, load_field(gf, UF), load_field(gf, PF), cf);
+  store_field(cf, CF);
+}
+
+void orc_multi_DQ(const int* Lf, const double* UF, const double* PF, const orc_multi* mp, double* DQ) {
+  MULTI_GEO;
+  std::vector<Mat> u = load_field(gf, UF), p = load_field(gf, PF), d(gc.V * 4);
+  Field g{&gf, u.data()}, m{&gf, p.data()};
+  for (long s = 0; s < gc.V; ++s) {
+    int xc[4], x[4];
+    gc.coord(s, xc);
+    for (int i = 0; i < 4; ++i) x[i] = 2 * xc[i];
+    for (int mu = 0; mu < 4; ++mu) d[s * 4 + mu] = get_DQ(g, m, x, mu, M.rho);
+  }
+  store_field(d, DQ);
+}
+
+void orc_multi_heatbath(const int* Lf, double* UC, const double* UF, const orc_multi* mp, uint64_t seed,
+                        uint64_t traj) {
+  MULTI_GEO;
+  std::vector<Mat> c = load_field(gc, UC);
+  heatbath(M, c, load_field(gf, UF), seed, traj);
+  store_field(c, UC);
+}
+
+void orc_multi_zeta_T(const int* Lf, double* UF, const double* PF, double* UC, double dt, int mag,
+                      const orc_multi* mp) {
+  MULTI_GEO;
+  std::vector<Mat> u = load_field(gf, UF), c = load_field(gc, UC), cm;
+  zeta_T(M, u, load_field(gf, PF), c, cm, dt, mag);
+  store_field(u, UF);
+  store_field(c, UC);
+}
+
+// which: 0 zeta_H, 1 md::force_gradient_integrator, 2 nested_integrator
+void orc_multi_integrate(const int* Lf, double* UF, double* PF, double* UC, const double* xi, const orc_action* a,
+                         const orc_multi* mp, int which, int steps, double dt, double* dev) {
+  MULTI_GEO;
+  std::vector<Mat> u = load_field(gf, UF), p = load_field(gf, PF), c = load_field(gc, UC);
+  std::vector<double> x = load_xi(gc, xi);
+  double d[2] = {0., 0.};
+  if (which == 0) zeta_H(M, u, p, c, x, *a, steps, dt, d);
+  else if (which == 1) multi_fg(M, u, p, c, x, *a, steps, dt, d);
+  else multi_nested(M, u, p, c, x, *a, steps, dt, d);
+  store_field(u, UF);
+  store_field(p, PF);
+  store_field(c, UC);
+  if (dev) { dev[0] = d[0]; dev[1] = d[1]; }
+}
+
+double orc_multi_hamiltonian(const int* Lf, const double* UF, const double* PF, const double* UC, const double* xi,
+                             const orc_action* a, const orc_multi* mp, double* parts) {
+  MULTI_GEO;
+  return multi_hamiltonian(M, load_field(gf, UF), load_field(gf, PF), load_field(gc, UC), load_xi(gc, xi), *a,
+                           parts);
+}
+
+// md::run_hmc_multi (field-multi-md.h:785-988) without the file output:
+// per trajectory init_momentum, heatbath of the coarse links, init_xi
+// (xi = XI0), H, zeta_H, H, Metropolis with forced accepts. A reject
+// restores both fields to the last accepted state (which also drops the
+// trajectory's heat-bath update, as in the reference).
+int orc_run_hmc_multi(const int* Lf, double* UF, double* UC, const orc_action* a, const orc_hmc* h,
+                      const orc_multi* mp, int first_traj, int ntraj, orc_multi_traj* out) {
+  MULTI_GEO;
+  std::vector<Mat> u = load_field(gf, UF), c = load_field(gc, UC), p;
+  std::vector<Mat> u_ext = u, c_ext = c;
+  std::vector<double> xi(gc.V * 4);
+  int naccept = 0;
+  for (int t = 0; t < ntraj; ++t) {
+    const int i = first_traj + t;
+    init_momentum(gf, h->seed, (uint64_t)i, 1., p);
+    heatbath(M, c, u, h->seed, (uint64_t)i);
+    std::fill(xi.begin(), xi.end(), mp->xi0);
+    double po[3], pn[3], dev[2];
+    const double h_old = multi_hamiltonian(M, u, p, c, xi, *a, po);
+    zeta_H(M, u, p, c, xi, *a, h->trajectory_length, h->dt, dev);
+    const double h_new = multi_hamiltonian(M, u, p, c, xi, *a, pn);
+    const double die = orc_uniform(h->seed, ST_ACCEPT, (uint64_t)i);
+    const double prob = std::exp(h_old - h_new);
+    bool acc = die < prob;
+    const bool bad = std::isnan(h_new) || std::isnan(h_old);
+    if (i < h->num_forced_accept && !bad) acc = true;
+    if (bad) acc = false;
+    if (acc) {
+      u_ext = u;
+      c_ext = c;
+      ++naccept;
+    } else {
+      u = u_ext;
+      c = c_ext;
+    }
+    if (out) {
+      orc_multi_traj& r = out[t];
+      r.h_old = h_old; r.h_new = h_new; r.dh = h_new - h_old; r.die_roll = die;
+      r.accept = acc ? 1 : 0;
+      Field ff{&gf, u.data()}, cf{&gc, c.data()};
+      r.plaq_fine = total_plaq(ff) / (18. * gf.V);
+      r.plaq_coarse = total_plaq(cf) / (18. * gc.V);
+      for (int k = 0; k < 3; ++k) { r.parts_old[k] = po[k]; r.parts_new[k] = pn[k]; }
+      r.reunit_dev[0] = dev[0]; r.reunit_dev[1] = dev[1];
+    }
+  }
+  store_field(u, UF);
+  store_field(c, UC);
+  return naccept;
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }

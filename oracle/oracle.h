@@ -42,6 +42,21 @@ typedef struct {
   double reunit_dev;        // max deviation reported by reunitarize
 } orc_traj;
 
+// Coupled coarse/fine md:: HMC (field-multi-md.h): rho, XI0 and md_alpha.
+typedef struct {
+  double rho;
+  double xi0;
+  double md_alpha;
+} orc_multi;
+
+typedef struct {
+  double h_old, h_new, dh, die_roll;
+  int accept;
+  double plaq_fine, plaq_coarse;   // avg_plaquette of both fields after the step
+  double parts_old[3], parts_new[3];  // kinetic, gauge, xi
+  double reunit_dev[2];            // fine, coarse
+} orc_multi_traj;
+
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_generators(double* out /* [8][18] */);
 void orc_expiQ(const double* Q, double* out, long n);
@@ -92,6 +107,25 @@ void orc_coarsen(const int* Lf, const double* Uf, int mag, double* Uc);
 void orc_update_constrain(const int* Lf, double* Uf, const double* Uc, int mag);
 double orc_constrained_plaquette(const int* L, const double* U, int mag);
 void orc_expand(const int* Lc, const double* Uc, int mag, double* Uf);
+// md:: coarse/fine HMC; Lf = fine extents (even), coarse = Lf / 2,
+// xi = [coarse site][mu] doubles.
+double orc_xi_energy(const int* Lf, const double* UF, const double* UC, const double* xi, const orc_multi* mp);
+void orc_multi_gb(const int* Lf, const double* UF, const orc_multi* mp, double* Gb);
+void orc_multi_Q(const int* Lf, const double* UF, const orc_multi* mp, double* Q);
+void orc_multi_fforce(const int* Lf, const double* UF, const double* UC, const double* xi, const orc_action* a,
+                      const orc_multi* mp, double* F);
+void orc_multi_cforce(const int* Lf, const double* UF, const double* PF, const orc_multi* mp, double* CF);
+void orc_multi_DQ(const int* Lf, const double* UF, const double* PF, const orc_multi* mp, double* DQ);
+void orc_multi_heatbath(const int* Lf, double* UC, const double* UF, const orc_multi* mp, uint64_t seed,
+                        uint64_t traj);
+void orc_multi_zeta_T(const int* Lf, double* UF, const double* PF, double* UC, double dt, int mag,
+                      const orc_multi* mp);
+void orc_multi_integrate(const int* Lf, double* UF, double* PF, double* UC, const double* xi, const orc_action* a,
+                         const orc_multi* mp, int which, int steps, double dt, double* dev);
+double orc_multi_hamiltonian(const int* Lf, const double* UF, const double* PF, const double* UC, const double* xi,
+                             const orc_action* a, const orc_multi* mp, double* parts);
+int orc_run_hmc_multi(const int* Lf, double* UF, double* UC, const orc_action* a, const orc_hmc* h,
+                      const orc_multi* mp, int first_traj, int ntraj, orc_multi_traj* out);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

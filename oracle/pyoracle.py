@@ -33,6 +33,16 @@ class Traj(C.Structure):
                 ("demon_e", C.c_double * 2), ("reunit_dev", C.c_double)]
 
 
+class Multi(C.Structure):
+    _fields_ = [("rho", C.c_double), ("xi0", C.c_double), ("md_alpha", C.c_double)]
+
+
+class MultiTraj(C.Structure):
+    _fields_ = [("h_old", C.c_double), ("h_new", C.c_double), ("dh", C.c_double), ("die_roll", C.c_double),
+                ("accept", C.c_int), ("plaq_fine", C.c_double), ("plaq_coarse", C.c_double),
+                ("parts_old", C.c_double * 3), ("parts_new", C.c_double * 3), ("reunit_dev", C.c_double * 2)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -95,6 +105,19 @@ def _declare(l):
         "orc_update_constrain": (None, [_I, _D, _D, C.c_int]),
         "orc_constrained_plaquette": (C.c_double, [_I, _D, C.c_int]),
         "orc_expand": (None, [_I, _D, C.c_int, _D]),
+        "orc_xi_energy": (C.c_double, [_I, _D, _D, _D, C.POINTER(Multi)]),
+        "orc_multi_gb": (None, [_I, _D, C.POINTER(Multi), _D]),
+        "orc_multi_Q": (None, [_I, _D, C.POINTER(Multi), _D]),
+        "orc_multi_fforce": (None, [_I, _D, _D, _D, C.POINTER(Action), C.POINTER(Multi), _D]),
+        "orc_multi_cforce": (None, [_I, _D, _D, C.POINTER(Multi), _D]),
+        "orc_multi_DQ": (None, [_I, _D, _D, C.POINTER(Multi), _D]),
+        "orc_multi_heatbath": (None, [_I, _D, _D, C.POINTER(Multi), C.c_uint64, C.c_uint64]),
+        "orc_multi_zeta_T": (None, [_I, _D, _D, _D, C.c_double, C.c_int, C.POINTER(Multi)]),
+        "orc_multi_integrate": (None, [_I, _D, _D, _D, _D, C.POINTER(Action), C.POINTER(Multi), C.c_int, C.c_int,
+                                       C.c_double, _D]),
+        "orc_multi_hamiltonian": (C.c_double, [_I, _D, _D, _D, _D, C.POINTER(Action), C.POINTER(Multi), _D]),
+        "orc_run_hmc_multi": (C.c_int, [_I, _D, _D, C.POINTER(Action), C.POINTER(Hmc), C.POINTER(Multi), C.c_int,
+                                        C.c_int, C.POINTER(MultiTraj)]),
         "orc_num_threads": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -334,6 +357,98 @@ def expand(Lc, Uc, mag):
     out = np.zeros((volume(Lf), 4, 18))
     lib().orc_expand(_dims(Lc), _dptr(Uc), mag, _dptr(out))
     return out
+
+
+# ------------------------------------------- md:: coarse/fine HMC (field-multi-md.h)
+# Fine extents L (even); coarse extents L/2; xi is (Vc, 4) doubles.
+def multi(rho=0.1, xi0=1.0, md_alpha=0.5):
+    return Multi(rho, xi0, md_alpha)
+
+
+def _coarse(L):
+    return [v // 2 for v in L]
+
+
+def xi_energy(L, U, Uc, xi, mp):
+    """get_xi_energy (field-multi-md.h:30-72)."""
+    return lib().orc_xi_energy(_dims(L), _dptr(U), _dptr(Uc), _dptr(np.ascontiguousarray(xi, dtype=np.float64)),
+                               C.byref(mp))
+
+
+def multi_gb(L, U, mp):
+    """get_Gb (field-multi-md.h:771-783): e^{iQ} U(2x) U(2x+mu) per coarse link."""
+    out = np.zeros((volume(_coarse(L)), 4, 18))
+    lib().orc_multi_gb(_dims(L), _dptr(U), C.byref(mp), _dptr(out))
+    return out
+
+
+def multi_Q(L, U, mp):
+    """get_Q (field-multi.h:33-39) at every coarse link."""
+    out = np.zeros((volume(_coarse(L)), 4, 18))
+    lib().orc_multi_Q(_dims(L), _dptr(U), C.byref(mp), _dptr(out))
+    return out
+
+
+def multi_fforce(L, U, Uc, xi, act, mp):
+    """md::get_Fforce (field-multi-md.h:116-368)."""
+    out = np.zeros((volume(L), 4, 18))
+    lib().orc_multi_fforce(_dims(L), _dptr(U), _dptr(Uc), _dptr(np.ascontiguousarray(xi, dtype=np.float64)),
+                           C.byref(act), C.byref(mp), _dptr(out))
+    return out
+
+
+def multi_cforce(L, U, P, mp):
+    """md::get_Cforce (field-multi-md.h:398-425)."""
+    out = np.zeros((volume(_coarse(L)), 4, 18))
+    lib().orc_multi_cforce(_dims(L), _dptr(U), _dptr(P), C.byref(mp), _dptr(out))
+    return out
+
+
+def multi_DQ(L, U, P, mp):
+    """md::get_DQ (field-multi-md.h:370-396) at every coarse link."""
+    out = np.zeros((volume(_coarse(L)), 4, 18))
+    lib().orc_multi_DQ(_dims(L), _dptr(U), _dptr(P), C.byref(mp), _dptr(out))
+    return out
+
+
+def multi_heatbath(L, Uc, U, mp, seed, traj):
+    """heatbath (field-multi-md.h:757-769), in place on Uc."""
+    lib().orc_multi_heatbath(_dims(L), _dptr(Uc), _dptr(U), C.byref(mp), seed, traj)
+
+
+def multi_zeta_T(L, U, P, Uc, dt, mag, mp):
+    """force_gradient_integrator_zeta_T (field-multi-md.h:441-475), in place."""
+    lib().orc_multi_zeta_T(_dims(L), _dptr(U), _dptr(P), _dptr(Uc), dt, mag, C.byref(mp))
+
+
+MULTI_ZETA_H, MULTI_FG, MULTI_NESTED = 0, 1, 2
+
+
+def multi_integrate(L, U, P, Uc, xi, act, mp, which, steps, dt):
+    """zeta_H (477-597), md::force_gradient_integrator (599-661) or
+    nested_integrator (663-695), in place; returns the reunitarize deviations."""
+    dev = np.zeros(2)
+    lib().orc_multi_integrate(_dims(L), _dptr(U), _dptr(P), _dptr(Uc),
+                              _dptr(np.ascontiguousarray(xi, dtype=np.float64)), C.byref(act), C.byref(mp), which,
+                              steps, dt, _dptr(dev))
+    return dev
+
+
+def multi_hamiltonian(L, U, P, Uc, xi, act, mp):
+    """md::get_hamiltonian_multi (field-multi-md.h:74-114): (H, [K, S, S_xi])."""
+    parts = np.zeros(3)
+    h = lib().orc_multi_hamiltonian(_dims(L), _dptr(U), _dptr(P), _dptr(Uc),
+                                    _dptr(np.ascontiguousarray(xi, dtype=np.float64)), C.byref(act), C.byref(mp),
+                                    _dptr(parts))
+    return h, parts
+
+
+def run_hmc_multi(L, U, Uc, act, h, mp, first_traj, ntraj):
+    """md::run_hmc_multi (field-multi-md.h:785-988) without file output, in place."""
+    out = (MultiTraj * ntraj)()
+    nacc = lib().orc_run_hmc_multi(_dims(L), _dptr(U), _dptr(Uc), C.byref(act), C.byref(h), C.byref(mp),
+                                   first_traj, ntraj, out)
+    return nacc, [out[i] for i in range(ntraj)]
 
 
 def num_threads():
