@@ -24,6 +24,7 @@
 // (serving batches do not run this path).
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -47,7 +48,10 @@ using D = double;
 using LV = LinkView<double, 18>;
 using AV = AlgView<double>;
 
-constexpr int NSLOT = 32;
+#ifndef MAG_MULTI_WAVES
+#define MAG_MULTI_WAVES 1
+#endif
+#define MAG_MULTI_OCC __attribute__((amdgpu_waves_per_eu(MAG_MULTI_WAVES)))
 
 struct MultiArgs {
   Geom gf, gc;
@@ -64,10 +68,21 @@ struct MultiArgs {
   double* part;
 };
 
-MAG_D void shift(const Geom& g, int* y, int d, int n) {
+template <int K>
+using IC = std::integral_constant<int, K>;
+
+// Shift of a fine coordinate along the compile-time direction K, wrapped
+// into [0, L) (|n| <= 2 < L).
+template <int K>
+MAG_D void shift(const Geom& g, int* y, int n) {
+  const int v = y[K] + n, L = g.L[K];
+  y[K] = v < 0 ? v + L : (v >= L ? v - L : v);
+}
+// the same for a direction known only at run time (wave-uniform)
+MAG_D void shift_rt(const Geom& g, int* y, int k, int n) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    if (i == d) {
+    if (i == k) {
       const int v = y[i] + n, L = g.L[i];
       y[i] = v < 0 ? v + L : (v >= L ? v - L : v);
     }
@@ -78,14 +93,51 @@ MAG_D int cidx(const Geom& gc, const int* y) {
   return (y[0] >> 1) + gc.S[1] * (y[1] >> 1) + gc.S[2] * (y[2] >> 1) + gc.S[3] * (y[3] >> 1);
 }
 MAG_D double xi_at(const MultiArgs& a, int mu, int sc) { return a.xi[(size_t)mu * a.gc.V + sc]; }
-// the j-th direction other than mu
-MAG_D int other(int mu, int j) { return j < mu ? j : j + 1; }
+
+// f(IC<NU>) for the three directions NU != MU
+template <int MU, class F>
+MAG_D void for_nu(F&& f) {
+  if constexpr (MU != 0) f(IC<0>{});
+  if constexpr (MU != 1) f(IC<1>{});
+  if constexpr (MU != 2) f(IC<2>{});
+  if constexpr (MU != 3) f(IC<3>{});
+}
+// Directions are compile-time inside every kernel body: the block-uniform
+// blockIdx.y selects a specialised body, so shifts and field-plane offsets
+// fold to constants.
+template <class F>
+MAG_D void with_mu(int mu, F&& f) {
+  switch (mu) {
+    case 0: f(IC<0>{}); break;
+    case 1: f(IC<1>{}); break;
+    case 2: f(IC<2>{}); break;
+    default: f(IC<3>{}); break;
+  }
+}
+// k = 3 mu + (index of nu among the directions other than mu)
+template <class F>
+MAG_D void with_mu_nu(int k, F&& f) {
+  switch (k) {
+    case 0: f(IC<0>{}, IC<1>{}); break;
+    case 1: f(IC<0>{}, IC<2>{}); break;
+    case 2: f(IC<0>{}, IC<3>{}); break;
+    case 3: f(IC<1>{}, IC<0>{}); break;
+    case 4: f(IC<1>{}, IC<2>{}); break;
+    case 5: f(IC<1>{}, IC<3>{}); break;
+    case 6: f(IC<2>{}, IC<0>{}); break;
+    case 7: f(IC<2>{}, IC<1>{}); break;
+    case 8: f(IC<2>{}, IC<3>{}); break;
+    case 9: f(IC<3>{}, IC<0>{}); break;
+    case 10: f(IC<3>{}, IC<1>{}); break;
+    default: f(IC<3>{}, IC<2>{}); break;
+  }
+}
 
 // Ordered product along a fine path (get_path_ordered_product_insertion,
 // field-matrix.h:177-200), starting at y.
 struct Walk {
-  const LV& u;
-  const Geom& g;
+  LV u;
+  Geom g;
   int y[4];
   M3<D> m;
   bool fresh = true;
@@ -98,13 +150,15 @@ struct Walk {
     else m = mul(m, a);
     fresh = false;
   }
-  MAG_D void fwd(int d) {
-    right(u.template load<D>(d, lin(g, y)));
-    shift(g, y, d, 1);
+  template <int K>
+  MAG_D void fwd() {
+    right(u.template load<D>(K, lin(g, y)));
+    shift<K>(g, y, 1);
   }
-  MAG_D void bwd(int d) {
-    shift(g, y, d, -1);
-    const M3<D> a = u.template load<D>(d, lin(g, y));
+  template <int K>
+  MAG_D void bwd() {
+    shift<K>(g, y, -1);
+    const M3<D> a = u.template load<D>(K, lin(g, y));
     if (fresh) m = dag(a);
     else m = mul_nd(m, a);
     fresh = false;
@@ -115,27 +169,27 @@ struct Walk {
 // i-weighted accumulation: t += (i w) m
 MAG_D void acc_i(M3<D>& t, double w, const M3<D>& m) { caxpy(t, cpx<D>{0., w}, m); }
 
-// Q of coarse link (c, mu) (as algebra components) and U2; fine x0 = 2c.
-MAG_D void coarse_Q(const MultiArgs& a, const int* x0, int mu, D q[8], M3<D>& U2) {
+// Q of coarse link (c, MU) (as algebra components) and U2; fine x0 = 2c.
+template <int MU>
+MAG_D void coarse_Q(const MultiArgs& a, const int* x0, D q[8], M3<D>& U2) {
   int y[4] = {x0[0], x0[1], x0[2], x0[3]};
-  const M3<D> U1 = a.U.template load<D>(mu, lin(a.gf, y));
-  shift(a.gf, y, mu, 1);
-  U2 = mul(U1, a.U.template load<D>(mu, lin(a.gf, y)));
+  const M3<D> U1 = a.U.template load<D>(MU, lin(a.gf, y));
+  shift<MU>(a.gf, y, 1);
+  U2 = mul(U1, a.U.template load<D>(MU, lin(a.gf, y)));
   M3<D> stp = zero3<D>();
-#pragma unroll 1
-  for (int j = 0; j < 3; ++j) {
-    const int nu = other(mu, j);
+  for_nu<MU>([&](auto nc) {
+    constexpr int NU = decltype(nc)::value;
     {  // nu, mu, mu, -nu
       Walk w(a.U, a.gf, x0);
-      w.fwd(nu); w.fwd(mu); w.fwd(mu); w.bwd(nu);
+      w.fwd<NU>(); w.fwd<MU>(); w.fwd<MU>(); w.bwd<NU>();
       add_to(stp, w.m);
     }
     {  // -nu, mu, mu, nu
       Walk w(a.U, a.gf, x0);
-      w.bwd(nu); w.fwd(mu); w.fwd(mu); w.fwd(nu);
+      w.bwd<NU>(); w.fwd<MU>(); w.fwd<MU>(); w.fwd<NU>();
       add_to(stp, w.m);
     }
-  }
+  });
   force_alg_from(mul_nd(stp, U2), q);  // -i TA(stp U2^+)
 #pragma unroll
   for (int k = 0; k < 8; ++k) q[k] *= a.rho;
@@ -148,21 +202,23 @@ MAG_D void coarse_site(const MultiArgs& a, int sc, int* x0) {
 }
 
 // Lambda (compute_Lambda with Sigma' = C^+, field-multi-md.h:164-176) and e^{iQ}.
-__global__ __launch_bounds__(BLOCK) void k_multi_prep(MultiArgs a) {
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_prep(MultiArgs a) {
   const int sc = blockIdx.x * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
-  const int mu = blockIdx.y;
   int x0[4];
   coarse_site(a, sc, x0);
-  D q[8];
-  M3<D> U2;
-  coarse_Q(a, x0, mu, q, U2);
-  const M3<D> X = mul_nd(U2, a.C.template load<D>(mu, sc));  // U2 Sigma'
-  D lam[8];
-  M3<D> eQ;
-  ch_lambda(alg_to_mat(q), X, lam, eQ);
-  a.lam.store(mu, sc, lam);
-  a.eq.store(mu, sc, eQ);
+  with_mu(blockIdx.y, [&](auto mc) {
+    constexpr int MU = decltype(mc)::value;
+    D q[8];
+    M3<D> U2;
+    coarse_Q<MU>(a, x0, q, U2);
+    const M3<D> X = mul_nd(U2, a.C.template load<D>(MU, sc));  // U2 Sigma'
+    D lam[8];
+    M3<D> eQ;
+    ch_lambda(alg_to_mat(q), X, lam, eQ);
+    a.lam.store(MU, sc, lam);
+    a.eq.store(MU, sc, eQ);
+  });
 }
 
 // ---------------------------------------------------------------- fine force
@@ -170,245 +226,271 @@ __global__ __launch_bounds__(BLOCK) void k_multi_prep(MultiArgs a) {
 // insertion position of each path, c(y) its coarse site.
 
 // class 1 (196-252): x = 2c
-MAG_D M3<D> fclass1(const MultiArgs& a, const int* x, int sc, int mu) {
+template <int MU>
+MAG_D M3<D> fclass1(const MultiArgs& a, const int* x, int sc) {
   const D r = a.rho;
   int s[4] = {x[0], x[1], x[2], x[3]};
-  shift(a.gf, s, mu, 1);
-  const D xm = xi_at(a, mu, sc);
-  const M3<D> Lm = a.lam.template load_mat<D>(mu, sc);
+  shift<MU>(a.gf, s, 1);
+  const D xm = xi_at(a, MU, sc);
+  const M3<D> Lm = a.lam.template load_mat<D>(MU, sc);
   // direct term U(x+mu) C^+ e^{iQ} xi
-  M3<D> t = scale(mul(mul_nd(a.U.template load<D>(mu, lin(a.gf, s)), a.C.template load<D>(mu, sc)),
-                      a.eq.template load<D>(mu, sc)),
+  M3<D> t = scale(mul(mul_nd(a.U.template load<D>(MU, lin(a.gf, s)), a.C.template load<D>(MU, sc)),
+                      a.eq.template load<D>(MU, sc)),
                   xm);
-#pragma unroll 1
-  for (int j = 0; j < 3; ++j) {
-    const int nu = other(mu, j);
+  for_nu<MU>([&](auto nc) {
+    constexpr int NU = decltype(nc)::value;
     {
       Walk w(a.U, a.gf, s);
-      w.fwd(mu); w.fwd(nu); w.bwd(mu); w.bwd(mu); w.bwd(nu); w.ins(Lm);
+      w.fwd<MU>(); w.fwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.bwd<NU>(); w.ins(Lm);
       acc_i(t, r * xm, w.m);
     }
     {
       Walk w(a.U, a.gf, s);
-      w.fwd(mu); w.bwd(nu); w.bwd(mu); w.bwd(mu); w.fwd(nu); w.ins(Lm);
+      w.fwd<MU>(); w.bwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.fwd<NU>(); w.ins(Lm);
       acc_i(t, r * xm, w.m);
     }
     {
       Walk w(a.U, a.gf, s);
-      w.fwd(nu); w.fwd(nu); w.bwd(mu); w.bwd(nu); w.bwd(nu); w.ins(a.lam.template load_mat<D>(nu, sc));
-      acc_i(t, -r * xi_at(a, nu, sc), w.m);
+      w.fwd<NU>(); w.fwd<NU>(); w.bwd<MU>(); w.bwd<NU>(); w.bwd<NU>(); w.ins(a.lam.template load_mat<D>(NU, sc));
+      acc_i(t, -r * xi_at(a, NU, sc), w.m);
     }
     {
       int y[4] = {x[0], x[1], x[2], x[3]};
-      shift(a.gf, y, nu, -2);
+      shift<NU>(a.gf, y, -2);
       const int cy = cidx(a.gc, y);
       Walk w(a.U, a.gf, s);
-      w.bwd(nu); w.bwd(nu); w.bwd(mu); w.ins(a.lam.template load_mat<D>(nu, cy)); w.fwd(nu); w.fwd(nu);
-      acc_i(t, r * xi_at(a, nu, cy), w.m);
+      w.bwd<NU>(); w.bwd<NU>(); w.bwd<MU>(); w.ins(a.lam.template load_mat<D>(NU, cy)); w.fwd<NU>(); w.fwd<NU>();
+      acc_i(t, r * xi_at(a, NU, cy), w.m);
     }
-  }
+  });
   return t;
 }
 
 // class 2 (254-278): x = 2c + nu
-MAG_D M3<D> fclass2(const MultiArgs& a, const int* x, int sc, int mu, int nu) {
+template <int MU, int NU>
+MAG_D M3<D> fclass2(const MultiArgs& a, const int* x, int sc) {
   const D r = a.rho;
   int s[4] = {x[0], x[1], x[2], x[3]};
-  shift(a.gf, s, mu, 1);
-  M3<D> t;
+  shift<MU>(a.gf, s, 1);
+  M3<D> t = zero3<D>();
   {
     Walk w(a.U, a.gf, s);
-    w.fwd(mu); w.bwd(nu); w.bwd(mu); w.bwd(mu); w.ins(a.lam.template load_mat<D>(mu, sc)); w.fwd(nu);
-    t = zero3<D>();
-    acc_i(t, -r * xi_at(a, mu, sc), w.m);
+    w.fwd<MU>(); w.bwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.ins(a.lam.template load_mat<D>(MU, sc)); w.fwd<NU>();
+    acc_i(t, -r * xi_at(a, MU, sc), w.m);
   }
   {
     int y[4] = {x[0], x[1], x[2], x[3]};
-    shift(a.gf, y, nu, 1);
+    shift<NU>(a.gf, y, 1);
     const int cy = cidx(a.gc, y);
     Walk w(a.U, a.gf, s);
-    w.fwd(mu); w.fwd(nu); w.bwd(mu); w.bwd(mu); w.ins(a.lam.template load_mat<D>(mu, cy)); w.bwd(nu);
-    acc_i(t, -r * xi_at(a, mu, cy), w.m);
+    w.fwd<MU>(); w.fwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.ins(a.lam.template load_mat<D>(MU, cy)); w.bwd<NU>();
+    acc_i(t, -r * xi_at(a, MU, cy), w.m);
   }
   return t;
 }
 
 // class 3 (279-332): x = 2c + mu
-MAG_D M3<D> fclass3(const MultiArgs& a, const int* x, int sc, int mu) {
+template <int MU>
+MAG_D M3<D> fclass3(const MultiArgs& a, const int* x, int sc) {
   const D r = a.rho;
   int s[4] = {x[0], x[1], x[2], x[3]};
-  shift(a.gf, s, mu, 1);
+  shift<MU>(a.gf, s, 1);
   int ym[4] = {x[0], x[1], x[2], x[3]};
-  shift(a.gf, ym, mu, -1);  // = 2c
-  const D xm = xi_at(a, mu, sc);
-  const M3<D> Lm = a.lam.template load_mat<D>(mu, sc);
-  const M3<D> Uym = a.U.template load<D>(mu, lin(a.gf, ym));
+  shift<MU>(a.gf, ym, -1);  // = 2c
+  const D xm = xi_at(a, MU, sc);
+  const M3<D> Lm = a.lam.template load_mat<D>(MU, sc);
+  const M3<D> Uym = a.U.template load<D>(MU, lin(a.gf, ym));
   // direct term C^+ e^{iQ} U(x-mu) xi
-  M3<D> t = scale(mul(mul_dn(a.C.template load<D>(mu, sc), a.eq.template load<D>(mu, sc)), Uym), xm);
+  M3<D> t = scale(mul(mul_dn(a.C.template load<D>(MU, sc), a.eq.template load<D>(MU, sc)), Uym), xm);
   int yp[4] = {x[0], x[1], x[2], x[3]};
-  shift(a.gf, yp, mu, 1);  // = 2(c + mu)
+  shift<MU>(a.gf, yp, 1);  // = 2(c + mu)
   const int cp = cidx(a.gc, yp);
-#pragma unroll 1
-  for (int j = 0; j < 3; ++j) {
-    const int nu = other(mu, j);
+  for_nu<MU>([&](auto nc) {
+    constexpr int NU = decltype(nc)::value;
     {
       Walk w(a.U, a.gf, s);
-      w.fwd(nu); w.bwd(mu); w.bwd(mu); w.bwd(nu); w.ins(Lm); w.right(Uym);
+      w.fwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.bwd<NU>(); w.ins(Lm); w.right(Uym);
       acc_i(t, r * xm, w.m);
     }
     {
       Walk w(a.U, a.gf, s);
-      w.bwd(nu); w.bwd(mu); w.bwd(mu); w.fwd(nu); w.ins(Lm); w.right(Uym);
+      w.bwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.fwd<NU>(); w.ins(Lm); w.right(Uym);
       acc_i(t, r * xm, w.m);
     }
     {
       Walk w(a.U, a.gf, s);
-      w.ins(a.lam.template load_mat<D>(nu, cp)); w.fwd(nu); w.fwd(nu); w.bwd(mu); w.bwd(nu); w.bwd(nu);
-      acc_i(t, r * xi_at(a, nu, cp), w.m);
+      w.ins(a.lam.template load_mat<D>(NU, cp)); w.fwd<NU>(); w.fwd<NU>(); w.bwd<MU>(); w.bwd<NU>(); w.bwd<NU>();
+      acc_i(t, r * xi_at(a, NU, cp), w.m);
     }
     {
       int y[4] = {yp[0], yp[1], yp[2], yp[3]};
-      shift(a.gf, y, nu, -2);
+      shift<NU>(a.gf, y, -2);
       const int cy = cidx(a.gc, y);
       Walk w(a.U, a.gf, s);
-      w.bwd(nu); w.bwd(nu); w.ins(a.lam.template load_mat<D>(nu, cy)); w.bwd(mu); w.fwd(nu); w.fwd(nu);
-      acc_i(t, -r * xi_at(a, nu, cy), w.m);
+      w.bwd<NU>(); w.bwd<NU>(); w.ins(a.lam.template load_mat<D>(NU, cy)); w.bwd<MU>(); w.fwd<NU>(); w.fwd<NU>();
+      acc_i(t, -r * xi_at(a, NU, cy), w.m);
     }
-  }
+  });
   return t;
 }
 
 // class 4 (333-357): x = 2c + mu + nu
-MAG_D M3<D> fclass4(const MultiArgs& a, const int* x, int sc, int mu, int nu) {
+template <int MU, int NU>
+MAG_D M3<D> fclass4(const MultiArgs& a, const int* x, int sc) {
   const D r = a.rho;
   int s[4] = {x[0], x[1], x[2], x[3]};
-  shift(a.gf, s, mu, 1);
+  shift<MU>(a.gf, s, 1);
   M3<D> t = zero3<D>();
   {
     Walk w(a.U, a.gf, s);
-    w.bwd(nu); w.bwd(mu); w.bwd(mu); w.ins(a.lam.template load_mat<D>(mu, sc)); w.fwd(nu); w.fwd(mu);
-    acc_i(t, -r * xi_at(a, mu, sc), w.m);
+    w.bwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.ins(a.lam.template load_mat<D>(MU, sc)); w.fwd<NU>(); w.fwd<MU>();
+    acc_i(t, -r * xi_at(a, MU, sc), w.m);
   }
   {
     int y[4] = {x[0], x[1], x[2], x[3]};
-    shift(a.gf, y, nu, 1);
-    shift(a.gf, y, mu, -1);
+    shift<NU>(a.gf, y, 1);
+    shift<MU>(a.gf, y, -1);
     const int cy = cidx(a.gc, y);
     Walk w(a.U, a.gf, s);
-    w.fwd(nu); w.bwd(mu); w.bwd(mu); w.ins(a.lam.template load_mat<D>(mu, cy)); w.bwd(nu); w.fwd(mu);
-    acc_i(t, -r * xi_at(a, mu, cy), w.m);
+    w.fwd<NU>(); w.bwd<MU>(); w.bwd<MU>(); w.ins(a.lam.template load_mat<D>(MU, cy)); w.bwd<NU>(); w.fwd<MU>();
+    acc_i(t, -r * xi_at(a, MU, cy), w.m);
   }
   return t;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_multi_fforce(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
-  if (sc >= a.gc.V) return;
-  const int slot = blockIdx.y;
-  int x[4];
-  coarse_site(a, sc, x);
-  int mu, nu = 0, cls;
-  if (slot < 4) {
-    cls = 1;
-    mu = slot;
-  } else if (slot < 8) {
-    cls = 3;
-    mu = slot - 4;
-    shift(a.gf, x, mu, 1);
-  } else if (slot < 20) {
-    cls = 2;
-    mu = (slot - 8) / 3;
-    nu = other(mu, (slot - 8) % 3);
-    shift(a.gf, x, nu, 1);
-  } else {
-    cls = 4;
-    mu = (slot - 20) / 3;
-    nu = other(mu, (slot - 20) % 3);
-    shift(a.gf, x, mu, 1);
-    shift(a.gf, x, nu, 1);
-  }
-  M3<D> t;
-  if (cls == 1) t = fclass1(a, x, sc, mu);
-  else if (cls == 3) t = fclass3(a, x, sc, mu);
-  else if (cls == 2) t = fclass2(a, x, sc, mu, nu);
-  else t = fclass4(a, x, sc, mu, nu);
-  // fx += i TA(U(x) mTemp) (363-365); i TA(W) has components -Im Tr(T_a W)
+// fx += i TA(U(x) mTemp) (363-365); i TA(W) has components -Im Tr(T_a W)
+template <int MU>
+MAG_D void add_stout_term(const MultiArgs& a, const int* x, const M3<D>& t) {
   const int s = lin(a.gf, x);
   D w[8], f[8];
-  force_alg_from(mul(a.U.template load<D>(mu, s), t), w);
-  a.F.load(mu, s, f);
+  force_alg_from(mul(a.U.template load<D>(MU, s), t), w);
+  a.F.load(MU, s, f);
 #pragma unroll
   for (int k = 0; k < 8; ++k) f[k] -= w[k];
-  a.F.store(mu, s, f);
+  a.F.store(MU, s, f);
+}
+
+// One kernel per class, so the two light classes (three quarters of the work
+// items) are not held to the register footprint of the heavy ones.
+// grid.y: class 1 and 3: mu; class 2 and 4: 3 mu + (index of nu among the
+// directions other than mu).
+template <int CLS>
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_fforce(MultiArgs a) {
+  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  if (sc >= a.gc.V) return;
+  int x[4];
+  coarse_site(a, sc, x);
+  if constexpr (CLS == 1 || CLS == 3) {
+    with_mu(blockIdx.y, [&](auto mc) {
+      constexpr int MU = decltype(mc)::value;
+      if constexpr (CLS == 1) {
+        add_stout_term<MU>(a, x, fclass1<MU>(a, x, sc));
+      } else {
+        shift<MU>(a.gf, x, 1);
+        add_stout_term<MU>(a, x, fclass3<MU>(a, x, sc));
+      }
+    });
+  } else {
+    with_mu_nu(blockIdx.y, [&](auto mc, auto nc) {
+      constexpr int MU = decltype(mc)::value, NU = decltype(nc)::value;
+      shift<NU>(a.gf, x, 1);
+      if constexpr (CLS == 2) {
+        add_stout_term<MU>(a, x, fclass2<MU, NU>(a, x, sc));
+      } else {
+        shift<MU>(a.gf, x, 1);
+        add_stout_term<MU>(a, x, fclass4<MU, NU>(a, x, sc));
+      }
+    });
+  }
 }
 
 // ---------------------------------------------------------------- coarse force
 // Loop product and its derivative along U' = i P U on every link
-// (get_path_ordered_product_leftD, field-matrix.h:138-175).
+// (get_path_ordered_product_leftD, field-matrix.h:138-175), carried as the
+// pair (m, d) = (product, derivative) of the path so far:
+//   forward link X = U:     d <- (d + i m P) X,        m <- m X
+//   backward link X = U^+:  d <- d X - i (m X) P,      m <- m X
+// (three 3x3 products per link).
 struct DWalk {
-  const MultiArgs& a;
+  LV u;
+  AV p;
+  Geom g;
   int y[4];
   M3<D> m, d;
   bool fresh = true;
-  MAG_D DWalk(const MultiArgs& a_, const int* x) : a(a_) {
+  MAG_D DWalk(const MultiArgs& a, const int* x) : u(a.U), p(a.P), g(a.gf) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) y[i] = x[i];
   }
-  MAG_D void step(const M3<D>& X, const M3<D>& dX) {
+  MAG_D void fwd(int K) {
+    const int s = lin(g, y);
+    const M3<D> X = u.template load<D>(K, s);
+    const M3<D> P = p.template load_mat<D>(K, s);
     if (fresh) {
       m = X;
-      d = dX;
+      d = cscale(mul_hn(P, X), cpx<D>{0., 1.});
       fresh = false;
-      return;
+    } else {
+      M3<D> mP = zero3<D>();
+      mul_acc_nh(mP, m, P);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d.e[k] = {d.e[k].r - mP.e[k].i, d.e[k].i + mP.e[k].r};
+      d = mul(d, X);
+      m = mul(m, X);
     }
-    M3<D> nd = mul(d, X);
-    mul_acc(nd, m, dX);
-    d = nd;
-    m = mul(m, X);
+    shift_rt(g, y, K, 1);
   }
-  MAG_D void fwd(int k) {
-    const int s = lin(a.gf, y);
-    const M3<D> X = a.U.template load<D>(k, s);
-    const M3<D> PX = mul_hn(a.P.template load_mat<D>(k, s), X);
-    step(X, cscale(PX, cpx<D>{0., 1.}));  // i P U
-    shift(a.gf, y, k, 1);
-  }
-  MAG_D void bwd(int k) {
-    shift(a.gf, y, k, -1);
-    const int s = lin(a.gf, y);
-    const M3<D> Uy = a.U.template load<D>(k, s);
-    const M3<D> UP = mul_dh(Uy, a.P.template load_mat<D>(k, s));
-    step(dag(Uy), cscale(UP, cpx<D>{0., -1.}));  // (i P U)^+ = -i U^+ P
+  MAG_D void bwd(int K) {
+    shift_rt(g, y, K, -1);
+    const int s = lin(g, y);
+    const M3<D> Uy = u.template load<D>(K, s);
+    const M3<D> P = p.template load_mat<D>(K, s);
+    if (fresh) {
+      m = dag(Uy);
+      d = cscale(mul_dh(Uy, P), cpx<D>{0., -1.});
+      fresh = false;
+    } else {
+      const M3<D> mX = mul_nd(m, Uy);
+      M3<D> mXP = zero3<D>();
+      mul_acc_nh(mXP, mX, P);
+      d = mul_nd(d, Uy);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d.e[k] = {d.e[k].r + mXP.e[k].i, d.e[k].i - mXP.e[k].r};
+      m = mX;
+    }
   }
 };
 
-__global__ __launch_bounds__(BLOCK) void k_multi_cforce(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
-  if (sc >= a.gc.V) return;
-  const int mu = blockIdx.y;
-  int x0[4];
-  coarse_site(a, sc, x0);
-  // Omega = stp U2^+ and its derivative, over the 12 rectangle loops (get_DQ)
-  M3<D> Om = zero3<D>(), dOm = zero3<D>();
+template <int MU>
+MAG_D void cforce_link(const MultiArgs& a, const int* x0, int sc) {
+  // -i TA of Omega = stp U2^+ and of its derivative, summed over the 12
+  // rectangle loops (get_DQ) as algebra components; the nu loop stays rolled
+  // (one nu's loops live at a time).
+  D q[8] = {0., 0., 0., 0., 0., 0., 0., 0.}, dq[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  auto add_loop = [&](const M3<D>& lm, const M3<D>& ld) {
+    D t[8], dt[8];
+    force_alg_from(lm, t);
+    force_alg_from(ld, dt);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      q[k] += t[k];
+      dq[k] += dt[k];
+    }
+  };
 #pragma unroll 1
   for (int j = 0; j < 3; ++j) {
-    const int nu = other(mu, j);
+    const int nu = j < MU ? j : j + 1;
     {
       DWalk w(a, x0);
-      w.fwd(nu); w.fwd(mu); w.fwd(mu); w.bwd(nu); w.bwd(mu); w.bwd(mu);
-      add_to(Om, w.m);
-      add_to(dOm, w.d);
+      w.fwd(nu); w.fwd(MU); w.fwd(MU); w.bwd(nu); w.bwd(MU); w.bwd(MU);
+      add_loop(w.m, w.d);
     }
     {
       DWalk w(a, x0);
-      w.bwd(nu); w.fwd(mu); w.fwd(mu); w.fwd(nu); w.bwd(mu); w.bwd(mu);
-      add_to(Om, w.m);
-      add_to(dOm, w.d);
+      w.bwd(nu); w.fwd(MU); w.fwd(MU); w.fwd(nu); w.bwd(MU); w.bwd(MU);
+      add_loop(w.m, w.d);
     }
   }
-  D q[8], dq[8];
-  force_alg_from(Om, q);
-  force_alg_from(dOm, dq);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     q[k] *= a.rho;
@@ -417,28 +499,28 @@ __global__ __launch_bounds__(BLOCK) void k_multi_cforce(MultiArgs a) {
   const M3<D> Q = alg_to_mat(q), DQ = alg_to_mat(dq);
   // U2 and its derivative
   DWalk w(a, x0);
-  w.fwd(mu);
-  w.fwd(mu);
+  w.fwd(MU);
+  w.fwd(MU);
   const M3<D> U2 = w.m, DU2 = w.d;
   // DexpiQ (field-multi.h:90-178); zero when c1 == 0, as in the reference
   const M3<D> Q2 = mul(Q, Q);
   D c0, c1;
   ch_invariants(Q, Q2, c0, c1);
   const CHC<D> c = ch_coef<true>(c0, c1);
+  // dE = Tr(Q DQ) B1 + Tr(Q^2 DQ) B2 + f1 DQ + f2 (DQ Q + Q DQ), with the
+  // B1, B2 terms folded into three coefficients of 1, Q, Q^2
   M3<D> E, dE;
   if (c.zero) {
     E = unit3<D>();
     dE = zero3<D>();
   } else {
     E = ch_combine(Q, Q2, c.f0, c.f1, c.f2);
-    const M3<D> B1 = ch_combine(Q, Q2, c.b10, c.b11, c.b12);
-    const M3<D> B2 = ch_combine(Q, Q2, c.b20, c.b21, c.b22);
     const cpx<D> t1 = trace_mul(Q, DQ), t2 = trace_mul(Q2, DQ);
+    dE = ch_combine(Q, Q2, t1 * c.b10 + t2 * c.b20, t1 * c.b11 + t2 * c.b21, t1 * c.b12 + t2 * c.b22);
     M3<D> QD = mul_hn(Q, DQ);
     mul_acc_nh(QD, DQ, Q);  // DQ Q + Q DQ
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
-      dE.e[k] = cfma(t1, B1.e[k], cfma(t2, B2.e[k], cfma(c.f1, DQ.e[k], c.f2 * QD.e[k])));
+    for (int k = 0; k < 9; ++k) dE.e[k] = cfma(c.f1, DQ.e[k], cfma(c.f2, QD.e[k], dE.e[k]));
   }
   // (DexpiQ U2 + e^{iQ} DU2) (e^{iQ} U2)^+, times -i alpha/(1+alpha)
   M3<D> V = mul(dE, U2);
@@ -448,50 +530,62 @@ __global__ __launch_bounds__(BLOCK) void k_multi_cforce(MultiArgs a) {
   force_alg_from(mul_nd(V, G), h);  // Re Tr(T_a (-i V G^+))
 #pragma unroll
   for (int k = 0; k < 8; ++k) h[k] *= a.kappa;
-  a.F.store(mu, sc, h);
+  a.F.store(MU, sc, h);
+}
+
+// one kernel per direction (its own register budget and stack frame)
+template <int MU>
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_cforce(MultiArgs a) {
+  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  if (sc >= a.gc.V) return;
+  int x0[4];
+  coarse_site(a, sc, x0);
+  cforce_link<MU>(a, x0, sc);
 }
 
 // ---------------------------------------------------------------- Gb, S_xi, heat bath
-MAG_D M3<D> coarse_Gb(const MultiArgs& a, const int* x0, int mu) {
+template <int MU>
+MAG_D M3<D> coarse_Gb(const MultiArgs& a, const int* x0) {
   D q[8];
   M3<D> U2;
-  coarse_Q(a, x0, mu, q, U2);
+  coarse_Q<MU>(a, x0, q, U2);
   return mul(expiQ(alg_to_mat(q)), U2);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_multi_gb(MultiArgs a) {
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_gb(MultiArgs a) {
   const int sc = blockIdx.x * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
-  const int mu = blockIdx.y;
   int x0[4];
   coarse_site(a, sc, x0);
-  a.out.store(mu, sc, coarse_Gb(a, x0, mu));
+  with_mu(blockIdx.y, [&](auto mc) {
+    constexpr int MU = decltype(mc)::value;
+    a.out.store(MU, sc, coarse_Gb<MU>(a, x0));
+  });
 }
 
-__global__ __launch_bounds__(BLOCK) void k_multi_xi(MultiArgs a) {
+// per-link terms xi Re Tr(C^+ Gb), one direction per grid row (summed by the
+// final reduction over the partial sums of all rows)
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_xi(MultiArgs a) {
   const int sc = blockIdx.x * BLOCK + threadIdx.x;
   double acc = 0.;
   if (sc < a.gc.V) {
     int x0[4];
     coarse_site(a, sc, x0);
-#pragma unroll 1
-    for (int mu = 0; mu < 4; ++mu)
-      acc += retrace_mul_nd(coarse_Gb(a, x0, mu), a.C.template load<D>(mu, sc)) * xi_at(a, mu, sc);
+    with_mu(blockIdx.y, [&](auto mc) {
+      constexpr int MU = decltype(mc)::value;
+      acc = retrace_mul_nd(coarse_Gb<MU>(a, x0), a.C.template load<D>(MU, sc)) * xi_at(a, MU, sc);
+    });
   }
   const double t = block_sum(acc);
-  if (threadIdx.x == 0) a.part[blockIdx.x] = t;
+  if (threadIdx.x == 0) a.part[blockIdx.y * gridDim.x + blockIdx.x] = t;
 }
 
 // simplest_metropolis with random_su3_from_su2(0.9) (field-multi-md.h:748-769,
 // field-matrix.h:36-65); Philox heat-bath stream keyed by (coarse link, traj).
-__global__ __launch_bounds__(BLOCK) void k_multi_heatbath(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
-  if (sc >= a.gc.V) return;
-  const int mu = blockIdx.y;
-  int x0[4];
-  coarse_site(a, sc, x0);
-  const M3<D> env = coarse_Gb(a, x0, mu);
-  const uint32_t link = (uint32_t)sc * 4u + (uint32_t)mu;
+template <int MU>
+MAG_D void heatbath_link(const MultiArgs& a, const int* x0, int sc) {
+  const M3<D> env = coarse_Gb<MU>(a, x0);
+  const uint32_t link = (uint32_t)sc * 4u + (uint32_t)MU;
   const u32x4 w0 = rng_draw(a.seed, RNG_HEATBATH, a.traj, link, 0);
   const u32x4 w1 = rng_draw(a.seed, RNG_HEATBATH, a.traj, link, 1);
   const u32x4 w2 = rng_draw(a.seed, RNG_HEATBATH, a.traj, link, 2);
@@ -509,10 +603,18 @@ __global__ __launch_bounds__(BLOCK) void k_multi_heatbath(MultiArgs a) {
     case 1: m(0, 0) = {a0, a3}; m(0, 2) = {a2, a1}; m(2, 0) = {-a2, a1}; m(2, 2) = {a0, -a3}; break;
     default: m(0, 0) = {a0, a3}; m(0, 1) = {a2, a1}; m(1, 0) = {-a2, a1}; m(1, 1) = {a0, -a3}; break;
   }
-  const M3<D> var = a.C.template load<D>(mu, sc);
+  const M3<D> var = a.C.template load<D>(MU, sc);
   const M3<D> nv = mul(var, m);
   const double diff = a.coeff * retrace(sub(mul_nd(nv, env), mul_nd(var, env)));
-  if (rng_u01(w2.z, w2.w) < exp(diff)) a.C.store(mu, sc, nv);
+  if (rng_u01(w2.z, w2.w) < exp(diff)) a.C.store(MU, sc, nv);
+}
+
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_heatbath(MultiArgs a) {
+  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  if (sc >= a.gc.V) return;
+  int x0[4];
+  coarse_site(a, sc, x0);
+  with_mu(blockIdx.y, [&](auto mc) { heatbath_link<decltype(mc)::value>(a, x0, sc); });
 }
 
 // ---------------------------------------------------------------- host side
@@ -602,7 +704,11 @@ int fforce(const Multi& m, void* F, void* U, void* C, const void* xi, const mag_
   a.eq = LV{(double*)m.wc->Eq, m.gc.V};
   a.F = AV{(double*)F, m.gf.V};
   if ((r = mlaunch(m, "multi_prep", k_multi_prep, cgrid(m, 4), a))) return r;
-  return mlaunch(m, "multi_fforce", k_multi_fforce, cgrid(m, NSLOT), a);
+  if ((r = mlaunch(m, "multi_fforce", k_multi_fforce<1>, cgrid(m, 4), a)) ||
+      (r = mlaunch(m, "multi_fforce", k_multi_fforce<3>, cgrid(m, 4), a)) ||
+      (r = mlaunch(m, "multi_fforce", k_multi_fforce<2>, cgrid(m, 12), a)))
+    return r;
+  return mlaunch(m, "multi_fforce", k_multi_fforce<4>, cgrid(m, 12), a);
 }
 
 // md::get_Cforce into CF (coarse alg8)
@@ -610,18 +716,23 @@ int cforce(const Multi& m, void* CF, void* U, void* P) {
   MultiArgs a = args(m, U, nullptr, nullptr);
   a.P = AV{(double*)P, m.gf.V};
   a.F = AV{(double*)CF, m.gc.V};
-  return mlaunch(m, "multi_cforce", k_multi_cforce, cgrid(m, 4), a);
+  int r;
+  if ((r = mlaunch(m, "multi_cforce", k_multi_cforce<0>, cgrid(m, 1), a)) ||
+      (r = mlaunch(m, "multi_cforce", k_multi_cforce<1>, cgrid(m, 1), a)) ||
+      (r = mlaunch(m, "multi_cforce", k_multi_cforce<2>, cgrid(m, 1), a)))
+    return r;
+  return mlaunch(m, "multi_cforce", k_multi_cforce<3>, cgrid(m, 1), a);
 }
 
 // sum of xi Re Tr(C^+ Gb) into scalar slot
 int xi_sum(const Multi& m, void* U, void* C, const void* xi, int slot) {
   const int nb = (m.gc.V + BLOCK - 1) / BLOCK;
   int r;
-  if ((r = ensure_part(m.c, (size_t)nb))) return r;
+  if ((r = ensure_part(m.c, 4 * (size_t)nb))) return r;
   MultiArgs a = args(m, U, C, xi);
   a.part = m.c->d_part;
-  if ((r = mlaunch(m, "multi_xi", k_multi_xi, dim3(nb), a))) return r;
-  return final_reduce(m.c, m.c->d_part, nb, 1, 0, slot);
+  if ((r = mlaunch(m, "multi_xi", k_multi_xi, dim3(nb, 4, 1), a))) return r;
+  return final_reduce(m.c, m.c->d_part, 4 * nb, 1, 0, slot);
 }
 
 int coarse_update(const Multi& m, void* C, void* Cm, double eps) {
