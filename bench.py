@@ -76,6 +76,8 @@ def parse():
                    help="gauge action (dbw2 with --L 64 --n-md 16 --wloop is the 64^4 DBW2 workload)")
     p.add_argument("--beta", type=float, default=None, help="default: 2.6 Iwasaki, 1.04 DBW2, 6.0 Wilson")
     p.add_argument("--wloop", action="store_true", help="measure a 4x4 Wilson loop after every trajectory")
+    p.add_argument("--multi", action="store_true",
+                   help="time the coupled coarse/fine md:: HMC (FP64) instead of the headline workload")
     p.add_argument("--no-reversibility", dest="reversibility", action="store_false",
                    help="skip the FP64 reversibility round trip after the timed run")
     return p.parse_args()
@@ -179,8 +181,84 @@ def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing, wloop=Fal
     return dt, res, ch, loops
 
 
+# Coupled coarse/fine md:: HMC: 3x3 products per coarse site of one fine
+# stout-term evaluation (4 links of class 1 and 3 with 63 products, 12 of
+# class 2 and 4 with 11) and of one coarse-force evaluation (4 links, 12
+# loops of 6 links at 3 products, the U2 pair, ~20 for DexpiQ and the tail).
+MULTI_FFORCE_PRODUCTS_CSITE = 8 * 63 + 24 * 11
+MULTI_CFORCE_PRODUCTS_CSITE = 4 * (12 * 6 * 3 + 2 * 3 + 20)
+
+
+def multi_main(a):
+    """run_hmc_multi (field-multi-md.h:785-988) at fine L^4 / coarse (L/2)^4,
+    FP64: forced-accept warmup trajectories, then `steps` timed trajectories
+    with HIP-event kernel timing. Prints one JSON line."""
+    import mag_amd as M
+    from mag_amd import multi_md as MD
+    ctx = M.Context(0)
+    L, n = a.L, a.n_md
+    kind, beta, c1 = ACTIONS[a.action]
+    beta = a.beta if a.beta is not None else beta
+    arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=beta, gauge=M.Gauge(kind, c1),
+                     num_forced_accept_step=a.warmup)
+    mp = MD.params(0.1, 1.0)
+    dims = (L, L, L, L)
+    ch = MD.MultiChain(ctx, dims, arg, mp, seed=a.seed)
+    UF, UC = ch.fields()
+    UF.haar(a.seed)
+    MD.get_Gb(UC, UF, mp)
+    print(f"bench --multi: fine {L}^4 / coarse {L // 2}^4, n = {n}, warmup {a.warmup}, steps {a.steps}",
+          file=sys.stderr, flush=True)
+    if a.warmup:
+        ch.run(a.warmup)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    res = ch.run(a.steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    fams = {}
+    for f in ("force_wilson", "force_iwasaki", "multi_prep", "multi_fforce", "multi_cforce", "multi_heatbath",
+              "multi_xi", "link_update", "mom_update", "copy", "kinetic", "action", "reunitarize", "reduce",
+              "mom_refresh", "fill"):
+        ms, nl = ctx.kernel_time(f)
+        if nl:
+            fams[f] = (ms, nl)
+    Vc = (L // 2) ** 4
+    # the fine stout term: 4 class launches per evaluation
+    ms, nl = fams["multi_fforce"]
+    per_eval_s = ms * 1e-3 / (nl / 4)
+    tflops = MULTI_FFORCE_PRODUCTS_CSITE * M_FLOPS * Vc / per_eval_s / 1e12
+    cms, cnl = fams["multi_cforce"]
+    c_tflops = MULTI_CFORCE_PRODUCTS_CSITE * M_FLOPS * Vc / (cms * 1e-3 / (cnl / 4)) / 1e12
+    act_name = {"iwasaki": "Iwasaki", "dbw2": "DBW2", "wilson": "Wilson"}[a.action]
+    out = {"metric": "md:: coarse/fine HMC trajectories/s (run_hmc_multi, FP64)", "value": round(a.steps / dt, 4),
+           "unit": "traj/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (Haar-random fine links, seed %d)" % a.seed,
+           "config": {"workload": f"fine {L}^4 / coarse {L // 2}^4 {act_name}(beta={beta}), rho=0.1, XI0=1, "
+                                  f"md_alpha=0.5, zeta_H tau=1 n={n}, heat bath every trajectory",
+                      "lattice": [L] * 4, "parallelism": "single GPU"},
+           "accept_rate": round(sum(r.accept for r in res) / len(res), 3),
+           "dh": [round(r.dh, 5) for r in res],
+           "kernel_ms_per_traj": {f: round(v[0] / a.steps, 3) for f, v in fams.items()},
+           "roofline": {"bound": "valu", "kernel": "multi_fforce (4 class launches)", "achieved": round(tflops, 3),
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+                        "traffic": None, "avg_eval_ms": round(per_eval_s * 1e3, 4)},
+           "coarse_force_tflops": round(c_tflops, 3)}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
+    if a.multi:
+        if a.gpus != 1:
+            print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)",
+                  file=sys.stderr)
+            sys.exit(2)
+        multi_main(a)
+        return
     if a.fp64_steps < 0:
         a.fp64_steps = a.steps
     if a.gpus != 1:

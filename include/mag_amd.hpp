@@ -1,6 +1,7 @@
 // mag_amd.hpp — C++ host API with the reference's entry-point names
 // (hummingtree/mag field-hmc.h, field-matrix.h, field-wloop.h,
-// field-demon.h, NERSC I/O of field-matrix.h), as thin inline wrappers over
+// field-demon.h, NERSC I/O of field-matrix.h, the md:: coarse/fine HMC of
+// field-multi-md.h), as thin inline wrappers over
 // the C ABI of mag_amd.h.
 // A caller of the reference replaces qlat::Field<cps::Matrix> by
 // mag::Field and keeps its call sequence:
@@ -66,10 +67,10 @@ class Field {
   mag_field get() const { return h_; }
   mag_ctx ctx() const { return ctx_; }
   long volume() const { return (long)dims_[0] * dims_[1] * dims_[2] * dims_[3]; }
-  // qlat layout [site][mu][3][3][re,im]
+  // qlat layout [site][mu][3][3][re,im] ([site][mu] for MAG_REAL1)
   void upload(const std::vector<double>& aos) { check(mag_field_upload_aos(h_, aos.data()), ctx_); }
   std::vector<double> download() const {
-    std::vector<double> a((size_t)volume() * 4 * 18);
+    std::vector<double> a((size_t)volume() * 4 * (kind_ == MAG_REAL1 ? 1 : 18));
     check(mag_field_download_aos(h_, a.data()), ctx_);
     return a;
   }
@@ -251,5 +252,110 @@ class Chain {
   mag_ctx ctx_;
   mag_chain h_ = nullptr;
 };
+
+// The coupled coarse/fine HMC of field-multi-md.h (namespace md:: there too).
+// Fine fields have even extents, coarse fields half of them; CxField is a
+// MAG_REAL1 field of the coarse shape. rho and XI0 are the includer's
+// constants in the reference; here they travel in mag_multi_params.
+namespace md {
+
+constexpr double md_alpha = 0.5;  // field-multi-md.h:28
+
+inline double get_xi_energy(const Field& FgField, const Field& CgField, const Field& CxField,
+                            const mag_multi_params& mp) {
+  double v = 0.;
+  check(mag_xi_energy(FgField.get(), CgField.get(), CxField.get(), &mp, &v), FgField.ctx());
+  return v;
+}
+inline double get_hamiltonian_multi(const Field& FgField, const Field& FmField, const Arg_chmc& Farg,
+                                    const Field& CgField, const Field& CxField, const mag_multi_params& mp,
+                                    std::vector<double>& part) {
+  mag_action a = Farg.action();
+  double h = 0.;
+  part.assign(3, 0.);
+  check(mag_multi_hamiltonian(FgField.get(), FmField.get(), CgField.get(), CxField.get(), &a, &mp, part.data(), &h),
+        FgField.ctx());
+  return h;
+}
+inline void get_Fforce(Field& FfField, const Field& FgField, const Field& CgField, const Arg_chmc& Farg,
+                       const Field& CxField, const mag_multi_params& mp) {
+  mag_action a = Farg.action();
+  check(mag_multi_fforce(FfField.get(), FgField.get(), CgField.get(), CxField.get(), &a, &mp), FgField.ctx());
+}
+inline void get_Cforce(Field& CfField, const Field& FgField, const Field& FmField, const mag_multi_params& mp) {
+  check(mag_multi_cforce(CfField.get(), FgField.get(), FmField.get(), &mp), FgField.ctx());
+}
+inline void get_Gb(Field& CGbField, const Field& FgField, const mag_multi_params& mp) {
+  check(mag_multi_gb(CGbField.get(), FgField.get(), &mp), FgField.ctx());
+}
+inline void heatbath(Field& CgField, const Field& FgField, const mag_multi_params& mp, uint64_t seed,
+                     uint64_t traj) {
+  check(mag_multi_heatbath(CgField.get(), FgField.get(), &mp, seed, traj), FgField.ctx());
+}
+inline void force_gradient_integrator_zeta_T(Field& FgField, const Field& FmField, Field& CgField, double dt,
+                                             const Arg_chmc& Farg, const mag_multi_params& mp) {
+  check(mag_multi_zeta_T(FgField.get(), FmField.get(), CgField.get(), dt, Farg.mag, &mp), FgField.ctx());
+}
+// returns the reunitarize deviations {fine, coarse}
+inline std::array<double, 2> force_gradient_integrator_zeta_H(Field& FgField, Field& FmField, Field& CgField,
+                                                              const Field& CxField, const Arg_chmc& Farg,
+                                                              const mag_multi_params& mp) {
+  mag_action a = Farg.action();
+  std::array<double, 2> dev{};
+  check(mag_multi_integrate(FgField.get(), FmField.get(), CgField.get(), CxField.get(), &a, &mp, MAG_MULTI_ZETA_H,
+                            Farg.trajectory_length, Farg.dt, dev.data()),
+        FgField.ctx());
+  return dev;
+}
+inline std::array<double, 2> force_gradient_integrator(Field& FgField, Field& FmField, const Arg_chmc& Farg,
+                                                       int steps, Field& CgField, const Field& CxField,
+                                                       const mag_multi_params& mp) {
+  mag_action a = Farg.action();
+  std::array<double, 2> dev{};
+  check(mag_multi_integrate(FgField.get(), FmField.get(), CgField.get(), CxField.get(), &a, &mp, MAG_MULTI_FG, steps,
+                            Farg.dt, dev.data()),
+        FgField.ctx());
+  return dev;
+}
+inline void nested_integrator(Field& FgField, Field& FmField, const Arg_chmc& Farg, Field& CgField,
+                              const Field& CxField, const mag_multi_params& mp) {
+  mag_action a = Farg.action();
+  check(mag_multi_integrate(FgField.get(), FmField.get(), CgField.get(), CxField.get(), &a, &mp, MAG_MULTI_NESTED,
+                            Farg.trajectory_length, Farg.dt, nullptr),
+        FgField.ctx());
+}
+
+// run_hmc_multi (field-multi-md.h:785) as a device-resident chain.
+class MultiChain {
+ public:
+  MultiChain(const Context& ctx, std::array<int, 4> fine_dims, const Arg_chmc& Farg, const mag_hmc_params& p,
+             const mag_multi_params& mp)
+      : ctx_(ctx.get()) {
+    mag_action a = Farg.action();
+    check(mag_multi_create(ctx_, fine_dims.data(), &a, &p, &mp, &h_), ctx_);
+  }
+  ~MultiChain() {
+    if (h_) mag_multi_destroy(h_);
+  }
+  MultiChain(const MultiChain&) = delete;
+  MultiChain& operator=(const MultiChain&) = delete;
+  // borrowed device fields (fine, coarse)
+  std::pair<mag_field, mag_field> fields() const {
+    mag_field f = nullptr, c = nullptr;
+    check(mag_multi_fields(h_, &f, &c), ctx_);
+    return {f, c};
+  }
+  std::vector<mag_multi_result> run(int ntraj) {
+    std::vector<mag_multi_result> r((size_t)ntraj);
+    check(mag_multi_run(h_, ntraj, r.data()), ctx_);
+    return r;
+  }
+
+ private:
+  mag_ctx ctx_;
+  mag_multi h_ = nullptr;
+};
+
+}  // namespace md
 
 }  // namespace mag
