@@ -194,6 +194,23 @@ int mag_chain_traj_index(mag_chain c, long* next_traj);
 int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
                              long acc_rej[2]);
 
+/* -------------------------------------------------------- local Metropolis
+ * algGCtrnHeatBath::run (alg_gchb.h:290-387) on the Wilson action, FP64
+ * links: num_iter iterations of nHits Metropolis hits per link with
+ * getRandomSU3BySubsu2(small) proposals (alg_gchb.h:32-96); a constrained
+ * link (transverse coordinates multiples of mag) below the last one of its
+ * coarse link is updated together with its successor (U1 eps, eps^+ U2).
+ * mag = 0 updates every link alone. After each iteration: reunitarize and
+ * record the average and constrained plaquettes (plaq, cons_plaq: num_iter
+ * values each, may be NULL). acc_rej = {accept, reject, acceptCons,
+ * rejectCons}. Order: the reference's sequential site order (one lane; the
+ * same decisions as the oracle) or 16 checkerboard classes per iteration
+ * (mu x parity of x_mu x parity of the transverse coordinate sum, whose
+ * updates share no staple; even extents). Philox stream (seed, link, iteration). */
+enum { MAG_ORDER_REFERENCE = 0, MAG_ORDER_CHECKERBOARD = 1 };
+int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nhits, double small, uint64_t seed,
+                         int order, long acc_rej[4], double* plaq, double* cons_plaq);
+
 /* ------------------------------------------------------ mag-blocked lattices
  * Coarse/fine tools of the constrained HMC (link classes of is_constrained,
  * field-hmc.h:309-337). The fine field's extents are mag times the coarse

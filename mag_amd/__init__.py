@@ -24,7 +24,7 @@ __all__ = [
     "demon_microcanonical", "run_chmc", "Arg_export", "export_config_nersc", "import_config_nersc",
     "nersc_write", "nersc_read", "total_plaq_async", "total_rectangular_async", "get_hamiltonian_async",
     "avg_wilson_loop_async", "coarsen_links", "update_constrain", "check_constrained_plaquette",
-    "lattice_expansion",
+    "lattice_expansion", "GchbArg", "AlgGCtrnHeatBath", "local_metropolis", "ORDER_REFERENCE", "ORDER_CHECKERBOARD",
 ]
 
 _D = C.POINTER(C.c_double)
@@ -323,6 +323,56 @@ def demon_microcanonical(U, nsweeps, E_max, seed, E=None):
     ar = (C.c_long * 2)()
     _check(_lib.lib().mag_demon_microcanonical(U.handle, nsweeps, E_max, seed, _dptr(e), ar), U.ctx.handle)
     return e, (ar[0], ar[1])
+
+
+# ----------------------------------------------------------------- local Metropolis
+ORDER_REFERENCE, ORDER_CHECKERBOARD = 0, 1
+
+
+def local_metropolis(U, beta, mag, num_iter, nhits, small, seed, order=ORDER_CHECKERBOARD):
+    """algGCtrnHeatBath::run (alg_gchb.h:290-387) on FP64 links: returns
+    (acc_rej[4] = accept, reject, acceptCons, rejectCons; average plaquette
+    per iteration; constrained plaquette per iteration)."""
+    acc = (C.c_long * 4)()
+    p, cp = np.zeros(max(num_iter, 1)), np.zeros(max(num_iter, 1))
+    _check(_lib.lib().mag_local_metropolis(U.handle, beta, mag, num_iter, nhits, small, seed, order, acc,
+                                           _dptr(p), _dptr(cp)), U.ctx.handle)
+    return list(acc), p[:num_iter], cp[:num_iter]
+
+
+@dataclass
+class GchbArg:
+    """gchbArg (alg_gchb.h:98-105)."""
+    mag: int = 0
+    numIter: int = 1
+    nHits: int = 1
+    small: float = 0.5
+
+
+class AlgGCtrnHeatBath:
+    """algGCtrnHeatBath (alg_gchb.h:107-387) over a device link field; the
+    Wilson beta is passed explicitly (GJP.Beta() in CPS)."""
+
+    def __init__(self, U, beta, arg, seed=1234, order=ORDER_CHECKERBOARD):
+        self.U, self.beta, self.arg, self.seed, self.order = U, beta, arg, seed, order
+        self.accept = self.reject = self.acceptCons = self.rejectCons = 0
+
+    def run(self):
+        """numIter iterations; returns the rows the reference prints:
+        (iteration, avgPlaq, avgConsPlaq)."""
+        a = self.arg
+        acc, p, cp = local_metropolis(self.U, self.beta, a.mag, a.numIter, a.nHits, a.small, self.seed, self.order)
+        self.accept += acc[0]
+        self.reject += acc[1]
+        self.acceptCons += acc[2]
+        self.rejectCons += acc[3]
+        return [(i, p[i], cp[i]) for i in range(a.numIter)]
+
+    def accept_rate(self):
+        """accpetRate (alg_gchb.h:264-276): (normal, constrained, total)."""
+        n, c = self.accept + self.reject, self.acceptCons + self.rejectCons
+        return (self.accept / n if n else 0.0, self.acceptCons / c if c else 0.0,
+                (self.accept + self.acceptCons) / (n + c) if n + c else 0.0)
 
 
 # ----------------------------------------------------------------- chain

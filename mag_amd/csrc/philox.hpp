@@ -15,7 +15,7 @@
 
 namespace mag {
 
-enum : uint32_t { RNG_MOMENTUM = 1, RNG_HAAR = 2, RNG_ACCEPT = 3, RNG_DEMON = 4, RNG_HEATBATH = 5 };
+enum : uint32_t { RNG_MOMENTUM = 1, RNG_HAAR = 2, RNG_ACCEPT = 3, RNG_DEMON = 4, RNG_HEATBATH = 5, RNG_LOCAL = 6 };
 
 struct u32x4 {
   uint32_t x, y, z, w;

@@ -247,6 +247,7 @@ struct Workspace {
   void* Uh[2] = {nullptr, nullptr};  // FP64 smeared fields for the Hamiltonian
   void* Lam = nullptr;  // alg8: Lambda of the coarse/fine HMC (coarse shape)
   void* Eq = nullptr;   // mat18: e^{iQ} of the coarse/fine HMC (coarse shape)
+  void* Cnt = nullptr;  // V doubles: per-thread hit counts of the local Metropolis
 
   ~Workspace() {
     for (auto& a : allocs) dev_free(c, a.first, a.second);
@@ -283,6 +284,7 @@ struct Workspace {
     return get(&W, mat_bytes());
   }
   int need_Z() { return get(&Z, mat_bytes()); }
+  int need_cnt() { return get(&Cnt, (size_t)g.V * sizeof(double)); }
   int need_multi() {
     int r = get(&Lam, alg_bytes());
     if (r) return r;

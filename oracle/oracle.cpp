@@ -1184,6 +1184,117 @@ double multi_hamiltonian(const Multi& M, const std::vector<Mat>& UF, const std::
   return K + S + X;
 }
 
+// ---------------------------------------------------------------- local Metropolis
+// algGCtrnHeatBath (alg_gchb.h:107-387): constrained local Metropolis on the
+// Wilson action. CPS's per-site LRG is replaced by a Philox stream keyed by
+// (link, iteration): hit h of the update of link (x, mu) takes draws 3h,
+// 3h+1, 3h+2: a, b, c, d, select, die, each mapped to [-1, 1) as LRG with
+// SetInterval(1, -1) does. A pair update draws from the stream of its first link.
+enum : uint32_t { ST_LOCAL = 6 };
+
+struct LocalDraws {
+  double u[6];
+};
+LocalDraws local_draws(uint64_t seed, uint64_t iter, uint32_t link, int hit) {
+  LocalDraws r;
+  for (int k = 0; k < 3; ++k) {
+    uint32_t w[4];
+    draw(seed, ST_LOCAL, iter, link, (uint32_t)(3 * hit + k), w);
+    r.u[2 * k] = -1. + 2. * u01(w[0], w[1]);
+    r.u[2 * k + 1] = -1. + 2. * u01(w[2], w[3]);
+  }
+  return r;
+}
+
+// getRandomSU3BySubsu2 (alg_gchb.h:32-96) from the uniforms u[0..4]
+Mat random_su3_by_subsu2(double small, const double* u) {
+  double a = small * u[0];
+  if (a > 0) a *= -1.;
+  a += 1.;
+  double b = u[1], c = u[2], d = u[3];
+  const double norm = std::sqrt((1. - a * a) / (b * b + c * c + d * d));
+  b *= norm; c *= norm; d *= norm;
+  const double select = u[4];
+  Mat m = Mat::zero();
+  if (select < -1. / 3.) {
+    m(0, 0) = cplx(a, b); m(0, 1) = cplx(c, d);
+    m(1, 0) = cplx(-c, d); m(1, 1) = cplx(a, -b);
+    m(2, 2) = cplx(1.);
+  } else if (select > 1. / 3.) {
+    m(0, 0) = cplx(1.);
+    m(1, 1) = cplx(a, b); m(1, 2) = cplx(c, d);
+    m(2, 1) = cplx(-c, d); m(2, 2) = cplx(a, -b);
+  } else {
+    m(0, 0) = cplx(a, b); m(0, 2) = cplx(c, d);
+    m(1, 1) = cplx(1.);
+    m(2, 0) = cplx(-c, d); m(2, 2) = cplx(a, -b);
+  }
+  return m;
+}
+
+struct LocalArgs {
+  double beta, small;
+  int nhits;
+  uint64_t seed, iter;
+  long acc[4];  // accept, reject, acceptCons, rejectCons
+};
+
+// metropolisKernel (alg_gchb.h:128-183)
+void metropolis_kernel(Mat& U, const Mat& sigma, LocalArgs& A, uint32_t link) {
+  const double fb = A.beta / 3.;
+  for (int hit = 0; hit < A.nhits; ++hit) {
+    LocalDraws r = local_draws(A.seed, A.iter, link, hit);
+    Mat eps = random_su3_by_subsu2(A.small, r.u);
+    const double oldS = -fb * retr(U * sigma);
+    Mat t1 = U * eps;
+    const double newS = -fb * retr(t1 * sigma);
+    double die = r.u[5];
+    if (die < 0.) die *= -1.;
+    if (die < std::exp(oldS - newS)) { U = t1; A.acc[0]++; }
+    else A.acc[1]++;
+  }
+}
+
+// metropolisKernelPProd (alg_gchb.h:185-254): U1 eps and eps^+ U2 together
+void metropolis_pair(Mat& U1, const Mat& s1, Mat& U2, const Mat& s2, LocalArgs& A, uint32_t link) {
+  const double fb = A.beta / 3.;
+  for (int hit = 0; hit < A.nhits; ++hit) {
+    LocalDraws r = local_draws(A.seed, A.iter, link, hit);
+    Mat eps = random_su3_by_subsu2(A.small, r.u);
+    double oldS = -fb * retr(U1 * s1);
+    oldS += -fb * retr(U2 * s2);
+    Mat t1 = U1 * eps, t2 = dagger(eps) * U2;
+    double newS = -fb * retr(t1 * s1);
+    newS += -fb * retr(t2 * s2);
+    double die = r.u[5];
+    if (die < 0.) die *= -1.;
+    if (die < std::exp(oldS - newS)) { U1 = t1; U2 = t2; A.acc[2]++; }
+    else A.acc[3]++;
+  }
+}
+
+// alg_gchb.h:117-127: the transverse coordinates are multiples of mag
+bool gchb_constrained(const int* x, int mu, int mag) {
+  bool c = true;
+  for (int i = 0; i < 4; ++i)
+    if (i != mu) c = c && (x[i] % mag == 0);
+  return c;
+}
+
+// One link's update in run() (alg_gchb.h:327-368).
+void local_update(Field& f, const int* x, int mu, int mag, LocalArgs& A) {
+  const uint32_t link = (uint32_t)(f.g->index(x) * 4 + mu);
+  if (mag > 0 && gchb_constrained(x, mu, mag) && x[mu] % mag != mag - 1) {
+    int y[4] = {x[0], x[1], x[2], x[3]};
+    y[mu]++;
+    Mat s1 = staple_dagger(f, x, mu), s2 = staple_dagger(f, y, mu);
+    metropolis_pair(f.at(x, mu), s1, f.at(y, mu), s2, A, link);
+  } else {
+    Mat s = staple_dagger(f, x, mu);
+    metropolis_kernel(f.at(x, mu), s, A, link);
+  }
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -1735,6 +1846,65 @@ int orc_run_hmc_multi(const int* Lf, double* UF, double* UC, const orc_action* a
   store_field(u, UF);
   store_field(c, UC);
   return naccept;
+}
+
+// algGCtrnHeatBath::run (alg_gchb.h:290-387) on the Wilson action: numIter
+// iterations; per iteration every direction, every site in the reference's
+// order (order 0; x0 fastest within mu) or in 16 checkerboard classes (order
+// 1: mu, then x_mu parity, then the parity of the transverse coordinate sum;
+// links of one class share no staple, so a class's updates commute), then
+// reunitarize and record the average and constrained plaquettes. mag = 0
+// updates every link alone. acc = {accept, reject, acceptCons, rejectCons}.
+void orc_local_metropolis(const int* L, double* U, double beta, int mag, int num_iter, int nhits, double small,
+                          uint64_t seed, int order, long* acc, double* plaq, double* cons_plaq) {
+  Geo g(L);
+  std::vector<Mat> u = load_field(g, U);
+  Field f{&g, u.data()};
+  LocalArgs A{beta, small, nhits, seed, 0, {0, 0, 0, 0}};
+  for (int it = 0; it < num_iter; ++it) {
+    A.iter = (uint64_t)it;
+    for (int mu = 0; mu < 4; ++mu) {
+      if (order == 0) {
+        for (long s = 0; s < g.V; ++s) {
+          int x[4];
+          g.coord(s, x);
+          local_update(f, x, mu, mag, A);
+        }
+      } else {
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            for (long s = 0; s < g.V; ++s) {
+              int x[4];
+              g.coord(s, x);
+              int tr = 0;
+              for (int i = 0; i < 4; ++i)
+                if (i != mu) tr += x[i];
+              if (x[mu] % 2 == a && tr % 2 == b) local_update(f, x, mu, mag, A);
+            }
+      }
+    }
+    reunitarize(u);
+    if (plaq) plaq[it] = total_plaq(f) / (18. * g.V);
+    if (cons_plaq) {
+      store_field(u, U);
+      cons_plaq[it] = mag > 0 ? orc_constrained_plaquette(L, U, mag) : 0.;
+    }
+  }
+  store_field(u, U);
+  for (int k = 0; k < 4; ++k) acc[k] = A.acc[k];
+}
+
+// One link's metropolisKernel hits with a fixed staple sum (test hook).
+void orc_metropolis_link(double* U, const double* sigma, double beta, int nhits, double small, uint64_t seed,
+                         uint64_t iter, uint32_t link, long* acc) {
+  Mat u, s;
+  u.load(U);
+  s.load(sigma);
+  LocalArgs A{beta, small, nhits, seed, iter, {0, 0, 0, 0}};
+  metropolis_kernel(u, s, A, link);
+  u.store(U);
+  acc[0] += A.acc[0];
+  acc[1] += A.acc[1];
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }

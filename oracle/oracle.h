@@ -126,6 +126,13 @@ double orc_multi_hamiltonian(const int* Lf, const double* UF, const double* PF, 
                              const orc_action* a, const orc_multi* mp, double* parts);
 int orc_run_hmc_multi(const int* Lf, double* UF, double* UC, const orc_action* a, const orc_hmc* h,
                       const orc_multi* mp, int first_traj, int ntraj, orc_multi_traj* out);
+// algGCtrnHeatBath::run (alg_gchb.h:290-387): order 0 = reference site
+// order, 1 = 16 checkerboard classes; acc[4] = accept, reject, acceptCons,
+// rejectCons; plaq / cons_plaq (may be NULL) get one value per iteration.
+void orc_local_metropolis(const int* L, double* U, double beta, int mag, int num_iter, int nhits, double small,
+                          uint64_t seed, int order, long* acc, double* plaq, double* cons_plaq);
+void orc_metropolis_link(double* U, const double* sigma, double beta, int nhits, double small, uint64_t seed,
+                         uint64_t iter, uint32_t link, long* acc);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

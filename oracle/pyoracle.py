@@ -118,6 +118,10 @@ def _declare(l):
         "orc_multi_hamiltonian": (C.c_double, [_I, _D, _D, _D, _D, C.POINTER(Action), C.POINTER(Multi), _D]),
         "orc_run_hmc_multi": (C.c_int, [_I, _D, _D, C.POINTER(Action), C.POINTER(Hmc), C.POINTER(Multi), C.c_int,
                                         C.c_int, C.POINTER(MultiTraj)]),
+        "orc_local_metropolis": (None, [_I, _D, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                        C.c_int, C.POINTER(C.c_long), _D, _D]),
+        "orc_metropolis_link": (None, [_D, _D, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64,
+                                       C.c_uint32, C.POINTER(C.c_long)]),
         "orc_num_threads": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -449,6 +453,23 @@ def run_hmc_multi(L, U, Uc, act, h, mp, first_traj, ntraj):
     nacc = lib().orc_run_hmc_multi(_dims(L), _dptr(U), _dptr(Uc), C.byref(act), C.byref(h), C.byref(mp),
                                    first_traj, ntraj, out)
     return nacc, [out[i] for i in range(ntraj)]
+
+
+# ------------------------------------ local Metropolis (alg_gchb.h:107-387)
+def local_metropolis(L, U, beta, mag, num_iter, nhits, small, seed, order=0):
+    """algGCtrnHeatBath::run in place on U; returns (acc[4], plaq[iter], cons_plaq[iter])."""
+    acc = (C.c_long * 4)()
+    p, cp = np.zeros(num_iter), np.zeros(num_iter)
+    lib().orc_local_metropolis(_dims(L), _dptr(U), beta, mag, num_iter, nhits, small, seed, order, acc,
+                               _dptr(p), _dptr(cp))
+    return list(acc), p, cp
+
+
+def metropolis_link(U, sigma, beta, nhits, small, seed, it, link, acc=None):
+    """metropolisKernel (alg_gchb.h:128-183) on one 3x3 link with a fixed staple sum, in place."""
+    acc = (C.c_long * 2)() if acc is None else acc
+    lib().orc_metropolis_link(_dptr(U), _dptr(sigma), beta, nhits, small, seed, it, link, acc)
+    return acc
 
 
 def num_threads():
