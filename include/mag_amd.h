@@ -194,6 +194,20 @@ int mag_chain_traj_index(mag_chain c, long* next_traj);
 int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
                              long acc_rej[2]);
 
+/* Checkerboard demon: demon_microcanonical as a parallel update with a demon
+ * per link. Ep, Er, Ec, Et are MAG_REAL1 fields of U's shape holding every
+ * link's plaquette, rectangle, chair and twist demon energies (in/out; start
+ * them at zero, or at any values inside the window). Each hit proposes
+ * random_su3_from_su2(0.9) U as the sequential demon does (same Philox
+ * stream) and is accepted iff every demon of that link stays inside
+ * (-E_max, E_max) after taking the action change. Per sweep and direction
+ * the links go in 128 classes (x_mu mod 2, x_nu mod 4 for the transverse nu)
+ * whose members share no staple path: the sum over links of S_k + E_k is
+ * conserved for the closed loops (plaquette, rectangle) and the result does
+ * not depend on thread order. Extents must be multiples of 4. acc_rej may be NULL. */
+int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, mag_field Ec, mag_field Et, int nsweeps,
+                           double E_max, uint64_t seed, long acc_rej[2]);
+
 /* -------------------------------------------------------- local Metropolis
  * algGCtrnHeatBath::run (alg_gchb.h:290-387) on the Wilson action, FP64
  * links: num_iter iterations of nHits Metropolis hits per link with

@@ -25,6 +25,7 @@ __all__ = [
     "nersc_write", "nersc_read", "total_plaq_async", "total_rectangular_async", "get_hamiltonian_async",
     "avg_wilson_loop_async", "coarsen_links", "update_constrain", "check_constrained_plaquette",
     "lattice_expansion", "GchbArg", "AlgGCtrnHeatBath", "local_metropolis", "ORDER_REFERENCE", "ORDER_CHECKERBOARD",
+    "demon_checkerboard",
 ]
 
 _D = C.POINTER(C.c_double)
@@ -323,6 +324,20 @@ def demon_microcanonical(U, nsweeps, E_max, seed, E=None):
     ar = (C.c_long * 2)()
     _check(_lib.lib().mag_demon_microcanonical(U.handle, nsweeps, E_max, seed, _dptr(e), ar), U.ctx.handle)
     return e, (ar[0], ar[1])
+
+
+def demon_checkerboard(U, nsweeps, E_max, seed, demons=None):
+    """Checkerboard demon (demon_microcanonical with a demon per link, 128
+    commuting classes per direction). demons: four REAL1 fields (plaquette,
+    rectangle, chair, twist energies per link), created at zero when None.
+    Returns (demons, (accepted, rejected))."""
+    if demons is None:
+        vol = int(np.prod(U.dims))
+        demons = [Field(U.ctx, U.dims, REAL1, F64).upload(np.zeros((vol, 4))) for _ in range(4)]
+    ar = (C.c_long * 2)()
+    _check(_lib.lib().mag_demon_checkerboard(U.handle, *[d.handle for d in demons], nsweeps, E_max, seed, ar),
+           U.ctx.handle)
+    return demons, (ar[0], ar[1])
 
 
 # ----------------------------------------------------------------- local Metropolis

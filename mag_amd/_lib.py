@@ -115,6 +115,7 @@ SIGNATURES = {
     "mag_chain_demon_state": [_P, _D],
     "mag_chain_traj_index": [_P, _L],
     "mag_demon_microcanonical": [_P, C.c_int, C.c_double, C.c_uint64, _D, _L],
+    "mag_demon_checkerboard": [_P, _P, _P, _P, _P, C.c_int, C.c_double, C.c_uint64, _L],
     "mag_local_metropolis": [_P, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, _L, _D,
                              _D],
     "mag_coarsen_links": [_P, C.c_int, _P],

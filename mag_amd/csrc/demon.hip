@@ -203,6 +203,130 @@ __global__ __launch_bounds__(64) void k_demon_sweep(LinkView<double, 18> U, Geom
   }
 }
 
+// ------------------------------------------------------------ checkerboard demon
+// demon_microcanonical with a demon per link (four energies per link, four
+// MAG_REAL1 fields), updated in parallel: per direction MU, 128 classes
+// (x_MU mod 2, then x_nu mod 4 for the transverse nu in increasing order).
+// Every staple path of a link holds its MU-links at offsets with |d_nu| <= 2
+// and d_MU in {-1, 0, 1}, so two links of one class never meet in a path:
+// the updates of a class commute and each moves its own action change into
+// its own demons. One thread per class member; hit counts in per-thread slots.
+struct DemonClassArgs {
+  LinkView<double, 18> U;
+  Geom g;
+  const DemonTable* tab;
+  double* E[4];   // per-link demon energies, [mu][site] each
+  double* cnt;    // [thread][2]: accept, reject
+  double emax;
+  uint64_t seed;
+  int sweep, a, cls;  // x_MU parity, transverse residues (base 4)
+};
+
+template <int MU>
+__global__ __launch_bounds__(BLOCK) void k_demon_class(DemonClassArgs A) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= A.g.V / 128) return;
+  int x[4], r = t, c = A.cls;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int step = (i == MU) ? 2 : 4;
+    const int h = r % (A.g.L[i] / step);
+    r /= A.g.L[i] / step;
+    if (i == MU) {
+      x[i] = 2 * h + A.a;
+    } else {
+      x[i] = 4 * h + (c & 3);
+      c >>= 2;
+    }
+  }
+  const int s = x[0] + A.g.S[1] * x[1] + A.g.S[2] * x[2] + A.g.S[3] * x[3];
+  // the four staple sums (same paths and kinds as the sequential demon)
+  M3<double> X[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) X[k] = zero3<double>();
+#pragma unroll 1
+  for (int q = 0; q < NPATH; ++q) {
+    int y[4] = {x[0], x[1], x[2], x[3]};
+    M3<double> m;
+    bool fresh = true;
+#pragma unroll 1
+    for (int j = 0; j < PLEN; ++j) {
+      const int d = A.tab->dirs[MU][q][j];
+      if (d < 0) break;
+      const int k = d & 3, n = d < 4 ? 1 : -1;
+      if (n < 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i == k) y[i] = (y[i] == 0) ? A.g.L[i] - 1 : y[i] - 1;
+      }
+      const int sy = y[0] + A.g.S[1] * y[1] + A.g.S[2] * y[2] + A.g.S[3] * y[3];
+      const M3<double> l = A.U.template load<double>(k, sy);
+      if (fresh) m = (n > 0) ? l : dag(l);
+      else m = (n > 0) ? mul(m, l) : mul_nd(m, l);
+      fresh = false;
+      if (n > 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i == k) y[i] = (y[i] == A.g.L[i] - 1) ? 0 : y[i] + 1;
+      }
+    }
+    const int kd = A.tab->kind[q];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k == kd) add_to(X[k], m);
+  }
+  // plaquette and rectangle sums are used daggered, chair and twist as they are
+  X[0] = dag(X[0]);
+  X[1] = dag(X[1]);
+  double E[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) E[k] = A.E[k][(size_t)MU * A.g.V + s];
+  M3<double> v = A.U.template load<double>(MU, s);
+  const uint32_t link = (uint32_t)s * 4u + (uint32_t)MU;
+  double acc = 0., rej = 0.;
+#pragma unroll 1
+  for (int n = 0; n < 2; ++n) {
+    const u32x4 w0 = rng_draw(A.seed, RNG_DEMON, (uint64_t)A.sweep, link, 3 * n);
+    const u32x4 w1 = rng_draw(A.seed, RNG_DEMON, (uint64_t)A.sweep, link, 3 * n + 1);
+    const u32x4 w2 = rng_draw(A.seed, RNG_DEMON, (uint64_t)A.sweep, link, 3 * n + 2);
+    const double a0 = 0.9 + (1. - 0.9) * rng_u01(w0.x, w0.y);
+    double a1 = -1. + 2. * rng_u01(w0.z, w0.w);
+    double a2 = -1. + 2. * rng_u01(w1.x, w1.y);
+    double a3 = -1. + 2. * rng_u01(w1.z, w1.w);
+    const double nrm = sqrt((1. - a0 * a0) / (a1 * a1 + a2 * a2 + a3 * a3));
+    a1 *= nrm;
+    a2 *= nrm;
+    a3 *= nrm;
+    M3<double> m = unit3<double>();
+    switch (w2.x % 3) {
+      case 0: m(1, 1) = {a0, a3}; m(1, 2) = {a2, a1}; m(2, 1) = {-a2, a1}; m(2, 2) = {a0, -a3}; break;
+      case 1: m(0, 0) = {a0, a3}; m(0, 2) = {a2, a1}; m(2, 0) = {-a2, a1}; m(2, 2) = {a0, -a3}; break;
+      default: m(0, 0) = {a0, a3}; m(0, 1) = {a2, a1}; m(1, 0) = {-a2, a1}; m(1, 1) = {a0, -a3}; break;
+    }
+    const M3<double> mv = mul(m, v);
+    double dS[4];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      dS[k] = -retrace_mul(mv, X[k]) + retrace_mul(v, X[k]);
+      ok = ok && (E[k] - dS[k] > -A.emax) && (E[k] - dS[k] < A.emax);
+    }
+    if (ok) {
+      v = mv;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) E[k] -= dS[k];
+      acc += 1.;
+    } else {
+      rej += 1.;
+    }
+  }
+  A.U.store(MU, s, v);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) A.E[k][(size_t)MU * A.g.V + s] = E[k];
+  A.cnt[(size_t)t * 2] += acc;
+  A.cnt[(size_t)t * 2 + 1] += rej;
+}
+
 }  // namespace
 
 extern "C" int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
@@ -239,5 +363,75 @@ extern "C" int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, 
   for (int k = 0; k < 4; ++k) E[k] = h.E[k];
   acc_rej[0] = (long)h.acc;
   acc_rej[1] = (long)h.rej;
+  return MAG_OK;
+}
+
+extern "C" int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, mag_field Ec, mag_field Et, int nsweeps,
+                                      double E_max, uint64_t seed, long acc_rej[2]) {
+  mag_field Ef[4] = {Ep, Er, Ec, Et};
+  if (!U || nsweeps < 0) return MAG_ERR_ARG;
+  mag_ctx c = U->ctx;
+  if (U->kind != MAG_LINK18 || U->prec != MAG_F64)
+    return fail(c, MAG_ERR_UNSUPPORTED, "the demon runs on FP64 18-real links");
+  for (int k = 0; k < 4; ++k) {
+    if (!Ef[k] || Ef[k]->kind != MAG_REAL1 || Ef[k]->ctx != c)
+      return fail(c, MAG_ERR_ARG, "demon energies are four MAG_REAL1 fields of the links' context");
+    for (int i = 0; i < 4; ++i)
+      if (Ef[k]->dims[i] != U->dims[i]) return fail(c, MAG_ERR_GEOMETRY, "demon fields differ in extents");
+  }
+  for (int i = 0; i < 4; ++i)
+    if (U->g.L[i] % 4) return fail(c, MAG_ERR_GEOMETRY, "the checkerboard demon needs extents that are multiples of 4");
+  if (c->rec) return fail(c, MAG_ERR_UNSUPPORTED, "the demon is not recordable");
+  const int nthr = U->g.V / 128;
+  int r;
+  Workspace* ws = workspace(c, U->g, MAG_F64, 18);
+  if ((r = ws->need_cnt())) return r;  // V doubles >= 2 per thread
+  double* cnt = (double*)ws->Cnt;
+  if ((r = fill_dev(c, cnt, 0., 2 * nthr))) return r;
+  DemonTable* ht = new DemonTable();
+  build_table(*ht);
+  DemonTable* dt = nullptr;
+  r = dev_alloc(c, (void**)&dt, sizeof(DemonTable));
+  if (!r && hipMemcpyAsync(dt, ht, sizeof(DemonTable), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    r = fail(c, MAG_ERR_DEVICE, "demon table upload failed");
+  DemonClassArgs A{};
+  A.U = LinkView<double, 18>{(double*)U->p, U->g.V};
+  A.g = U->g;
+  A.tab = dt;
+  for (int k = 0; k < 4; ++k) A.E[k] = (double*)Ef[k]->p;
+  A.cnt = cnt;
+  A.emax = E_max;
+  A.seed = seed;
+  void (*ks[4])(DemonClassArgs) = {k_demon_class<0>, k_demon_class<1>, k_demon_class<2>, k_demon_class<3>};
+  const dim3 grid((nthr + BLOCK - 1) / BLOCK);
+  for (int sw = 0; sw < nsweeps && !r; ++sw) {
+    A.sweep = sw;
+    for (int mu = 0; mu < 4 && !r; ++mu)
+      for (int cl = 0; cl < 128 && !r; ++cl) {
+        A.a = cl & 1;
+        A.cls = cl >> 1;
+        KTimer t(c, "demon_checkerboard");
+        hipLaunchKernelGGL(ks[mu], grid, dim3(BLOCK), 0, c->stream, A);
+        if (hipGetLastError() != hipSuccess) r = fail(c, MAG_ERR_DEVICE, "demon launch failed");
+      }
+  }
+  std::vector<double> h((size_t)2 * nthr);
+  if (!r && (hipMemcpyAsync(h.data(), cnt, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+                 hipSuccess ||
+             hipStreamSynchronize(c->stream) != hipSuccess))
+    r = fail(c, MAG_ERR_DEVICE, "demon download failed");
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(c, dt, dt ? sizeof(DemonTable) : 0);
+  delete ht;
+  if (r) return r;
+  if (acc_rej) {
+    double a = 0., b = 0.;
+    for (int t = 0; t < nthr; ++t) {
+      a += h[(size_t)2 * t];
+      b += h[(size_t)2 * t + 1];
+    }
+    acc_rej[0] = (long)a;
+    acc_rej[1] = (long)b;
+  }
   return MAG_OK;
 }

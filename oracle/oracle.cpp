@@ -1907,6 +1907,76 @@ void orc_metropolis_link(double* U, const double* sigma, double beta, int nhits,
   acc[1] += A.acc[1];
 }
 
+// Checkerboard demon: demon_microcanonical (field-demon.h:25-126) with a
+// demon per link. Each link carries its own four energies E[(s*4+mu)*4+k]
+// (plaquette, rectangle, chair, twist); an accepted hit moves the link's
+// action change into them, within (-E_max, E_max) per demon. Per sweep and
+// direction mu the links are visited in 128 classes (x_mu mod 2, then x_nu
+// mod 4 for the transverse nu in increasing order) and in site order within a
+// class; members of one class share no staple path, so their updates commute.
+// Same proposals and Philox stream as the sequential demon.
+void orc_demon_checkerboard(const int* L, double* U, int nsweeps, double E_max, uint64_t seed, double* Elink,
+                            long* acc_rej) {
+  Geo g(L);
+  std::vector<Mat> u = load_field(g, U);
+  Field f{&g, u.data()};
+  long acc = 0, rej = 0;
+  for (int sw = 0; sw < nsweeps; ++sw)
+    for (int mu = 0; mu < DIMN; ++mu)
+      for (int cls = 0; cls < 128; ++cls)
+        for (long s = 0; s < g.V; ++s) {
+          int x[4];
+          g.coord(s, x);
+          int c = x[mu] % 2, sh = 1;
+          for (int i = 0; i < 4; ++i)
+            if (i != mu) {
+              c += (x[i] % 4) * 2 * sh;
+              sh *= 4;
+            }
+          if (c != cls) continue;
+          Mat& v = f.at(s, mu);
+          Mat X[4] = {staple_dagger(f, x, mu), rectangular_dagger(f, x, mu), chair_staple(f, x, mu),
+                      twist_staple(f, x, mu)};
+          double* E = Elink + (s * 4 + mu) * 4;
+          for (int n = 0; n < 2; ++n) {
+            uint32_t w0[4], w1[4], w2[4];
+            const uint32_t link = (uint32_t)(s * 4 + mu);
+            draw(seed, ST_DEMON, (uint64_t)sw, link, 3 * n, w0);
+            draw(seed, ST_DEMON, (uint64_t)sw, link, 3 * n + 1, w1);
+            draw(seed, ST_DEMON, (uint64_t)sw, link, 3 * n + 2, w2);
+            double a0 = 0.9 + (1. - 0.9) * u01(w0[0], w0[1]);
+            double a1 = -1. + 2. * u01(w0[2], w0[3]);
+            double a2 = -1. + 2. * u01(w1[0], w1[1]);
+            double a3 = -1. + 2. * u01(w1[2], w1[3]);
+            double nrm = std::sqrt((1. - a0 * a0) / (a1 * a1 + a2 * a2 + a3 * a3));
+            a1 *= nrm; a2 *= nrm; a3 *= nrm;
+            Mat m = Mat::unit();
+            switch (w2[0] % 3) {
+              case 0: m(1, 1) = cplx(a0, a3); m(1, 2) = cplx(a2, a1); m(2, 1) = cplx(-a2, a1); m(2, 2) = cplx(a0, -a3); break;
+              case 1: m(0, 0) = cplx(a0, a3); m(0, 2) = cplx(a2, a1); m(2, 0) = cplx(-a2, a1); m(2, 2) = cplx(a0, -a3); break;
+              case 2: m(0, 0) = cplx(a0, a3); m(0, 1) = cplx(a2, a1); m(1, 0) = cplx(-a2, a1); m(1, 1) = cplx(a0, -a3); break;
+            }
+            Mat mv = m * v;
+            double dS[4];
+            bool ok = true;
+            for (int q = 0; q < 4; ++q) {
+              dS[q] = -retr(mv * X[q]) + retr(v * X[q]);
+              ok = ok && (E[q] - dS[q] > -E_max) && (E[q] - dS[q] < E_max);
+            }
+            if (ok) {
+              v = mv;
+              for (int q = 0; q < 4; ++q) E[q] -= dS[q];
+              ++acc;
+            } else {
+              ++rej;
+            }
+          }
+        }
+  store_field(u, U);
+  acc_rej[0] = acc;
+  acc_rej[1] = rej;
+}
+
 int orc_num_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -133,6 +133,10 @@ void orc_local_metropolis(const int* L, double* U, double beta, int mag, int num
                           uint64_t seed, int order, long* acc, double* plaq, double* cons_plaq);
 void orc_metropolis_link(double* U, const double* sigma, double beta, int nhits, double small, uint64_t seed,
                          uint64_t iter, uint32_t link, long* acc);
+// Checkerboard demon with a demon per link: Elink [site][mu][4] in/out
+// (plaquette, rectangle, chair, twist); 128 commuting classes per direction.
+void orc_demon_checkerboard(const int* L, double* U, int nsweeps, double E_max, uint64_t seed, double* Elink,
+                            long* acc_rej);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -122,6 +122,7 @@ def _declare(l):
                                         C.c_int, C.POINTER(C.c_long), _D, _D]),
         "orc_metropolis_link": (None, [_D, _D, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_uint64,
                                        C.c_uint32, C.POINTER(C.c_long)]),
+        "orc_demon_checkerboard": (None, [_I, _D, C.c_int, C.c_double, C.c_uint64, _D, C.POINTER(C.c_long)]),
         "orc_num_threads": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -470,6 +471,15 @@ def metropolis_link(U, sigma, beta, nhits, small, seed, it, link, acc=None):
     acc = (C.c_long * 2)() if acc is None else acc
     lib().orc_metropolis_link(_dptr(U), _dptr(sigma), beta, nhits, small, seed, it, link, acc)
     return acc
+
+
+def demon_checkerboard(L, U, nsweeps, E_max, seed, Elink=None):
+    """Checkerboard demon with per-link demons, in place on U and Elink
+    ((V, 4, 4): site, mu, {plaquette, rectangle, chair, twist}); returns (Elink, (acc, rej))."""
+    Elink = np.zeros((volume(L), 4, 4)) if Elink is None else Elink
+    ar = (C.c_long * 2)()
+    lib().orc_demon_checkerboard(_dims(L), _dptr(U), nsweeps, E_max, seed, _dptr(Elink), ar)
+    return Elink, (ar[0], ar[1])
 
 
 def num_threads():

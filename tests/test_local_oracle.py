@@ -118,3 +118,37 @@ def test_strong_coupling(orc):
     U = orc.haar_links(L4, 61)
     _, plaq, _ = orc.local_metropolis(L4, U, beta, 0, 40, 4, 1.0, 62, 1)
     assert abs(plaq[10:].mean() - beta / 18) < 0.006, plaq[10:].mean()
+
+
+# ------------------------------------------------------ checkerboard demon
+def test_checkerboard_demon_conservation(orc):
+    """Per-link demons: sum S_k + E_k conserved for the plaquette and
+    rectangle sums, every demon inside its window."""
+    L = (4, 4, 4, 8)
+    U = orc.haar_links(L, 131)
+    s0 = (-orc.total_plaq(L, U), -orc.total_rectangular(L, U))
+    E, (acc, rej) = orc.demon_checkerboard(L, U, 2, 1.5, 132)
+    s1 = (-orc.total_plaq(L, U), -orc.total_rectangular(L, U))
+    assert acc > 0 and rej > 0
+    assert abs(s1[0] + E[:, :, 0].sum() - s0[0]) < 1e-9
+    assert abs(s1[1] + E[:, :, 1].sum() - s0[1]) < 1e-9
+    assert np.abs(E).max() < 1.5
+    # a second call continues from the demons it is given
+    E2, _ = orc.demon_checkerboard(L, U, 1, 1.5, 133, E.copy())
+    s2 = (-orc.total_plaq(L, U), -orc.total_rectangular(L, U))
+    assert abs(s2[0] + E2[:, :, 0].sum() - s0[0]) < 1e-9
+
+
+def test_checkerboard_demon_unbounded_window_equals_sequential(orc):
+    """With an unbounded window every proposal is accepted, so each link ends
+    as m2 m1 U whatever the visiting order: the checkerboard and the
+    sequential demon give the same links, and the same plaquette and
+    rectangle demon totals (minus the action change)."""
+    L = (4, 4, 4, 4)
+    U0 = orc.haar_links(L, 141)
+    a, b = U0.copy(), U0.copy()
+    E, (acc, rej) = orc.demon_checkerboard(L, a, 1, 1e300, 142)
+    Es, (acc_s, rej_s) = orc.demon_microcanonical(L, b, 1, 1e300, 142)
+    assert rej == 0 and rej_s == 0 and acc == acc_s == 2 * 4 * 256
+    assert np.abs(a - b).max() < 1e-13
+    assert abs(E[:, :, 0].sum() - Es[0]) < 1e-9 and abs(E[:, :, 1].sum() - Es[1]) < 1e-9
