@@ -76,6 +76,8 @@ def parse():
                    help="gauge action (dbw2 with --L 64 --n-md 16 --wloop is the 64^4 DBW2 workload)")
     p.add_argument("--beta", type=float, default=None, help="default: 2.6 Iwasaki, 1.04 DBW2, 6.0 Wilson")
     p.add_argument("--wloop", action="store_true", help="measure a 4x4 Wilson loop after every trajectory")
+    p.add_argument("--local", action="store_true",
+                   help="time the local updates: checkerboard Metropolis and checkerboard demon (FP64)")
     p.add_argument("--multi", action="store_true",
                    help="time the coupled coarse/fine md:: HMC (FP64) instead of the headline workload")
     p.add_argument("--no-reversibility", dest="reversibility", action="store_false",
@@ -250,8 +252,77 @@ def multi_main(a):
     print(json.dumps(out))
 
 
+def local_main(a):
+    """Local updates at L^4, FP64 Wilson beta = 5.7: algGCtrnHeatBath in the
+    checkerboard order (nHits = 10, small = 0.5, mag = 0 and 2) and the
+    checkerboard demon (2 hits per link, E_max = 2). Hot start; `warmup`
+    untimed iterations / sweeps, then `steps` timed ones. Prints one JSON line."""
+    import numpy as np
+    import mag_amd as M
+    ctx = M.Context(0)
+    L = a.L
+    dims = (L, L, L, L)
+    nlinks = 4 * L ** 4
+    out = {"metric": "local-update link updates/s (checkerboard Metropolis and demon, FP64)", "unit": "link hits/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (Haar-random links, seed %d)" % a.seed,
+           "config": {"workload": f"{L}^4 Wilson beta=5.7", "lattice": [L] * 4, "parallelism": "single GPU"}}
+    res = {}
+    for mag in (0, 2):
+        f = M.Field(ctx, dims, M.LINK18, M.F64).haar(a.seed)
+        if a.warmup:
+            M.local_metropolis(f, 5.7, mag, a.warmup, 10, 0.5, a.seed)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        ctx.reset_timing()
+        t0 = time.perf_counter()
+        acc, p, _ = M.local_metropolis(f, 5.7, mag, a.steps, 10, 0.5, a.seed + 1)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        kms, kn = ctx.kernel_time("local_metropolis")
+        ctx.set_timing(False)
+        hits = sum(acc)
+        res[f"metropolis_mag{mag}"] = {"hits_per_s": round(hits / dt, 1), "ms_per_iteration": round(1e3 * dt / a.steps, 3),
+                                       "kernel_ms_per_iteration": round(kms / a.steps, 3),
+                                       "accept": round((acc[0] + acc[2]) / max(hits, 1), 4),
+                                       "plaquette": round(float(p[-1]), 6)}
+    f = M.Field(ctx, dims, M.LINK18, M.F64).haar(a.seed)
+    d, _ = M.demon_checkerboard(f, max(a.warmup // 5, 1), 2.0, a.seed)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    nsw = max(a.steps // 5, 1)
+    t0 = time.perf_counter()
+    d, (acc, rej) = M.demon_checkerboard(f, nsw, 2.0, a.seed + 1, d)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kms, kn = ctx.kernel_time("demon_checkerboard")
+    ctx.set_timing(False)
+    res["demon_checkerboard"] = {"hits_per_s": round((acc + rej) / dt, 1), "ms_per_sweep": round(1e3 * dt / nsw, 3),
+                                 "kernel_ms_per_sweep": round(kms / nsw, 3), "accept": round(acc / (acc + rej), 4),
+                                 "sweeps": nsw}
+    out["value"] = res["metropolis_mag0"]["hits_per_s"]
+    out["ms_per_step"] = res["metropolis_mag0"]["ms_per_iteration"]
+    # roofline of the Metropolis class kernel: per link 12 staple products +
+    # 10 hits x 3 products (U eps, two traces counted as one product each)
+    prods = 12 + 10 * 3
+    tflops = prods * M_FLOPS * nlinks / (res["metropolis_mag0"]["kernel_ms_per_iteration"] * 1e-3) / 1e12
+    out["roofline"] = {"bound": "valu", "kernel": "local_class (checkerboard Metropolis)", "achieved": round(tflops, 3),
+                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+                       "traffic": None}
+    out["results"] = res
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
+    if a.local:
+        if a.gpus != 1:
+            print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)",
+                  file=sys.stderr)
+            sys.exit(2)
+        local_main(a)
+        return
     if a.multi:
         if a.gpus != 1:
             print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)",

@@ -226,6 +226,40 @@ inline mag_nersc_header import_config_nersc(Field& U, const std::string& path) {
   return h;
 }
 
+// algGCtrnHeatBath (alg_gchb.h:107-387): gchbArg plus the Wilson beta (GJP.Beta() in CPS)
+struct gchbArg {
+  int mag = 0;
+  int numIter = 1;
+  int nHits = 1;
+  double small = 0.5;
+};
+struct LocalResult {
+  long accept = 0, reject = 0, acceptCons = 0, rejectCons = 0;
+  std::vector<double> avgPlaq, avgConsPlaq;  // per iteration
+};
+inline LocalResult local_metropolis(Field& U, double beta, const gchbArg& arg, uint64_t seed,
+                                    int order = MAG_ORDER_CHECKERBOARD) {
+  LocalResult r;
+  long a[4] = {0, 0, 0, 0};
+  r.avgPlaq.assign((size_t)arg.numIter, 0.);
+  r.avgConsPlaq.assign((size_t)arg.numIter, 0.);
+  check(mag_local_metropolis(U.get(), beta, arg.mag, arg.numIter, arg.nHits, arg.small, seed, order, a,
+                             r.avgPlaq.data(), r.avgConsPlaq.data()),
+        U.ctx());
+  r.accept = a[0];
+  r.reject = a[1];
+  r.acceptCons = a[2];
+  r.rejectCons = a[3];
+  return r;
+}
+// demon_microcanonical with a demon per link (MAG_REAL1 fields of U's shape)
+inline std::pair<long, long> demon_checkerboard(Field& U, Field& Ep, Field& Er, Field& Ec, Field& Et, int nsweeps,
+                                                double E_max, uint64_t seed) {
+  long ar[2] = {0, 0};
+  check(mag_demon_checkerboard(U.get(), Ep.get(), Er.get(), Ec.get(), Et.get(), nsweeps, E_max, seed, ar), U.ctx());
+  return {ar[0], ar[1]};
+}
+
 // run_chmc (field-hmc.h:770) as a device-resident chain.
 class Chain {
  public:

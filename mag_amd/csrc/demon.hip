@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) void k_demon_sweep(LinkView<double, 18> U, Geom
 // Every staple path of a link holds its MU-links at offsets with |d_nu| <= 2
 // and d_MU in {-1, 0, 1}, so two links of one class never meet in a path:
 // the updates of a class commute and each moves its own action change into
-// its own demons. One thread per class member; hit counts in per-thread slots.
+// its own demons. One wavefront per class member; hit counts in per-member slots.
 struct DemonClassArgs {
   LinkView<double, 18> U;
   Geom g;
@@ -222,10 +222,13 @@ struct DemonClassArgs {
   int sweep, a, cls;  // x_MU parity, transverse residues (base 4)
 };
 
+// One wavefront per class member: lane q sums paths q, q + 64, q + 128 of
+// the link, a butterfly over the 64 lanes (fixed order) reduces the four
+// 3x3 sums, and lane 0 makes the hits.
 template <int MU>
-__global__ __launch_bounds__(BLOCK) void k_demon_class(DemonClassArgs A) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
-  if (t >= A.g.V / 128) return;
+__global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x;
   int x[4], r = t, c = A.cls;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -240,12 +243,13 @@ __global__ __launch_bounds__(BLOCK) void k_demon_class(DemonClassArgs A) {
     }
   }
   const int s = x[0] + A.g.S[1] * x[1] + A.g.S[2] * x[2] + A.g.S[3] * x[3];
-  // the four staple sums (same paths and kinds as the sequential demon)
+  // this lane's share of the four staple sums (same paths and kinds as the
+  // sequential demon)
   M3<double> X[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) X[k] = zero3<double>();
 #pragma unroll 1
-  for (int q = 0; q < NPATH; ++q) {
+  for (int q = lane; q < NPATH; q += 64) {
     int y[4] = {x[0], x[1], x[2], x[3]};
     M3<double> m;
     bool fresh = true;
@@ -275,6 +279,17 @@ __global__ __launch_bounds__(BLOCK) void k_demon_class(DemonClassArgs A) {
     for (int k = 0; k < 4; ++k)
       if (k == kd) add_to(X[k], m);
   }
+  // butterfly over the wavefront: every lane ends with the full sums
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        X[k].e[e].r += __shfl_xor(X[k].e[e].r, off, 64);
+        X[k].e[e].i += __shfl_xor(X[k].e[e].i, off, 64);
+      }
+  if (lane != 0) return;
   // plaquette and rectangle sums are used daggered, chair and twist as they are
   X[0] = dag(X[0]);
   X[1] = dag(X[1]);
@@ -403,7 +418,7 @@ extern "C" int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, m
   A.emax = E_max;
   A.seed = seed;
   void (*ks[4])(DemonClassArgs) = {k_demon_class<0>, k_demon_class<1>, k_demon_class<2>, k_demon_class<3>};
-  const dim3 grid((nthr + BLOCK - 1) / BLOCK);
+  const dim3 grid(nthr);  // one wavefront per class member
   for (int sw = 0; sw < nsweeps && !r; ++sw) {
     A.sweep = sw;
     for (int mu = 0; mu < 4 && !r; ++mu)
@@ -411,7 +426,7 @@ extern "C" int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, m
         A.a = cl & 1;
         A.cls = cl >> 1;
         KTimer t(c, "demon_checkerboard");
-        hipLaunchKernelGGL(ks[mu], grid, dim3(BLOCK), 0, c->stream, A);
+        hipLaunchKernelGGL(ks[mu], grid, dim3(64), 0, c->stream, A);
         if (hipGetLastError() != hipSuccess) r = fail(c, MAG_ERR_DEVICE, "demon launch failed");
       }
   }
