@@ -77,7 +77,9 @@ enum ActionKind : int { ACT_WILSON = 0, ACT_IWASAKI = 1 };
 // U_nu(x+mu)^+ (plaquette, 1x2 tall, 2x1 backward) are summed before the
 // last multiply, as are the three that end in U_nu(x-nu+mu); the 2x1 forward
 // rectangles of every plane share the trailing U_mu(x+mu)^+, applied once
-// after the nu loop. 62 instead of 85 3x3 products per link.
+// after the nu loop. 62 instead of 85 3x3 products per link. Every partial
+// path is a product of links, so it is formed in two rows (mul_*_r2) and
+// completed once (reconstruct_row2) before it enters a sum.
 template <int ACT, class TS, int NC, class TC>
 MAG_D M3<TC> staple_sum(const LinkView<TS, NC>& u, const Geom& g, const Site& p, int mu, TC c0, TC c1) {
   M3<TC> S = zero3<TC>();
@@ -91,62 +93,60 @@ MAG_D M3<TC> staple_sum(const LinkView<TS, NC>& u, const Geom& g, const Site& p,
     const int opn = off_rt(g, p, nu, 1), omn = off_rt(g, p, nu, -1);
     const int xpn = p.s + opn, xmn = p.s + omn;
     if (ACT == ACT_WILSON) {
-      {
-        M3<TC> A = mul(u.template load<TC>(nu, p.s), u.template load<TC>(mu, xpn));
-        mul_nd_acc(S, A, u.template load<TC>(nu, xpm));
-      }
-      {
-        M3<TC> B = mul_dn(u.template load<TC>(nu, xmn), u.template load<TC>(mu, xmn));
-        mul_acc(S, B, u.template load<TC>(nu, xpm + omn));
-      }
+      // U_nu(x) U_mu(x+nu) U_nu(x+mu)^+
+      add_su3_r2(S, mul_nd_r2(mul_r2(u.template load_r2<TC>(nu, p.s), u.template load<TC>(mu, xpn)),
+                              u.template load<TC>(nu, xpm)));
+      // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu)
+      add_su3_r2(S, mul_r2(mul_dn_r2(u.template load<TC>(nu, xmn), u.template load<TC>(mu, xmn)),
+                           u.template load<TC>(nu, xpm + omn)));
     } else {
       const int op2n = off_rt(g, p, nu, 2), om2n = off_rt(g, p, nu, -2);
-      // ---- upper side of the plane
+      // ---- upper side of the plane: the paths that end in U_nu(x+mu)^+
+      //   G = c0 U_nu(x) U_mu(x+nu)                                (plaquette)
+      //     + c1 U_nu(x) U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+    (tall rectangle)
+      //     + c1 U_mu(x-mu)^+ U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu)  (backward rectangle)
+      // each an SU(3) product formed in two rows; S += G U_nu(x+mu)^+.
       {
+        const M3<TC> Un_x = u.template load_r2<TC>(nu, p.s);
         const M3<TC> Um_xpn = u.template load<TC>(mu, xpn);
-        // T3 = U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+
-        M3<TC> T = mul(u.template load<TC>(nu, xpn), u.template load<TC>(mu, p.s + op2n));
-        T = mul_nd(T, u.template load<TC>(nu, xpm + opn));
-        // tmp = c0 U_mu(x+nu) + c1 T3
-        M3<TC> tmp = scale(Um_xpn, c0);
-        axpy(tmp, c1, T);
-        const M3<TC> Un_x = u.template load<TC>(nu, p.s);
-        M3<TC> G = mul(Un_x, tmp);
-        // R1' = U_nu(x) U_mu(x+nu) U_mu(x+nu+mu) U_nu(x+2mu)^+
+        const M3<TC> P = mul_r2(Un_x, Um_xpn);
+        // forward rectangle R1' = P U_mu(x+nu+mu) U_nu(x+2mu)^+
+        add_su3_r2(R, mul_nd_r2(mul_r2(P, u.template load<TC>(mu, xpm + opn)), u.template load<TC>(nu, xp2m)));
+        M3<TC> G = zero3<TC>();
+        axpy_su3_r2(G, c0, P);
         {
-          M3<TC> A = mul(Un_x, Um_xpn);
-          A = mul(A, u.template load<TC>(mu, xpm + opn));
-          mul_nd_acc(R, A, u.template load<TC>(nu, xp2m));
+          M3<TC> T = mul_r2(Un_x, u.template load<TC>(nu, xpn));
+          T = mul_r2(T, u.template load<TC>(mu, p.s + op2n));
+          axpy_su3_r2(G, c1, mul_nd_r2(T, u.template load<TC>(nu, xpm + opn)));
         }
-        // T5 = U_mu(x-mu)^+ U_nu(x-mu) U_mu(x-mu+nu); G += c1 T5 U_mu(x+nu)
         {
-          M3<TC> T5 = mul_dn(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm));
-          T5 = scale(mul(T5, u.template load<TC>(mu, xmm + opn)), c1);
-          mul_acc(G, T5, Um_xpn);
+          M3<TC> T = mul_dn_r2(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm));
+          T = mul_r2(T, u.template load<TC>(mu, xmm + opn));
+          axpy_su3_r2(G, c1, mul_r2(T, Um_xpn));
         }
         mul_nd_acc(S, G, u.template load<TC>(nu, xpm));
       }
-      // ---- lower side of the plane
+      // ---- lower side: the paths that end in U_nu(x-nu+mu)
+      //   G = c0 U_nu(x-nu)^+ U_mu(x-nu)
+      //     + c1 U_nu(x-nu)^+ U_nu(x-2nu)^+ U_mu(x-2nu) U_nu(x-2nu+mu)
+      //     + c1 U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_mu(x-nu)
       {
-        const M3<TC> Um_xmn = u.template load<TC>(mu, xmn);
-        // T4 = U_nu(x-2nu)^+ U_mu(x-2nu) U_nu(x-2nu+mu)
-        M3<TC> T = mul_dn(u.template load<TC>(nu, p.s + om2n), u.template load<TC>(mu, p.s + om2n));
-        T = mul(T, u.template load<TC>(nu, xpm + om2n));
-        M3<TC> tmp = scale(Um_xmn, c0);
-        axpy(tmp, c1, T);
         const M3<TC> Un_xmn = u.template load<TC>(nu, xmn);
-        M3<TC> G = mul_dn(Un_xmn, tmp);
-        // R2' = U_nu(x-nu)^+ U_mu(x-nu) U_mu(x-nu+mu) U_nu(x-nu+2mu)
+        const M3<TC> Um_xmn = u.template load<TC>(mu, xmn);
+        const M3<TC> P = mul_dn_r2(Un_xmn, Um_xmn);
+        // forward rectangle R2' = P U_mu(x-nu+mu) U_nu(x-nu+2mu)
+        add_su3_r2(R, mul_r2(mul_r2(P, u.template load<TC>(mu, xpm + omn)), u.template load<TC>(nu, xp2m + omn)));
+        M3<TC> G = zero3<TC>();
+        axpy_su3_r2(G, c0, P);
         {
-          M3<TC> B = mul_dn(Un_xmn, Um_xmn);
-          B = mul(B, u.template load<TC>(mu, xpm + omn));
-          mul_acc(R, B, u.template load<TC>(nu, xp2m + omn));
+          M3<TC> T = mul_dd_r2(Un_xmn, u.template load<TC>(nu, p.s + om2n));
+          T = mul_r2(T, u.template load<TC>(mu, p.s + om2n));
+          axpy_su3_r2(G, c1, mul_r2(T, u.template load<TC>(nu, xpm + om2n)));
         }
-        // T6 = U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu); G += c1 T6 U_mu(x-nu)
         {
-          M3<TC> T6 = mul_dd(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm + omn));
-          T6 = scale(mul(T6, u.template load<TC>(mu, xmm + omn)), c1);
-          mul_acc(G, T6, Um_xmn);
+          M3<TC> T = mul_dd_r2(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm + omn));
+          T = mul_r2(T, u.template load<TC>(mu, xmm + omn));
+          axpy_su3_r2(G, c1, mul_r2(T, Um_xmn));
         }
         mul_acc(S, G, u.template load<TC>(nu, xpm + omn));
       }

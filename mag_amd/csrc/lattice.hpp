@@ -121,6 +121,20 @@ struct LinkView {
     if (NC == 12) reconstruct_row2(m);
     return m;
   }
+  // rows 0 and 1 only (row 2 zero), for a factor read only there
+  template <class TC>
+  MAG_D M3<TC> load_r2(int mu, int s) const {
+    const unsigned u = (unsigned)s;
+    M3<TC> m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      m.e[k].r = TC(at(plane(p, (size_t)(mu * NC + 2 * k), V), u));
+      m.e[k].i = TC(at(plane(p, (size_t)(mu * NC + 2 * k + 1), V), u));
+    }
+#pragma unroll
+    for (int k = 6; k < 9; ++k) m.e[k] = {TC(0), TC(0)};
+    return m;
+  }
   template <class TC>
   MAG_D void store(int mu, int s, const M3<TC>& m) const {
     const unsigned u = (unsigned)s;

@@ -413,12 +413,100 @@ MAG_HD M3<T> trless_antiherm(const M3<T>& m) {
 template <class T>
 MAG_HD void herm_traceless_alg(const M3<T>& m, T h[8]) { mat_to_alg(m, h); }
 
+// Two-row products. A product of SU(3) links is in SU(3), so only its rows 0
+// and 1 need forming (two thirds of the multiply-adds); reconstruct_row2
+// completes it when the third row is used. Row 2 of the result is zero.
+// The first factor of mul_r2 / mul_nd_r2 is read only in rows 0 and 1.
+// C = A B, rows 0-1
+template <class T>
+MAG_HD M3<T> mul_r2(const M3<T>& a, const M3<T>& b) {
+  M3<T> c;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = a(i, 0) * b(0, j);
+      s = cfma(a(i, 1), b(1, j), s);
+      s = cfma(a(i, 2), b(2, j), s);
+      c(i, j) = s;
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) c(2, j) = {T(0), T(0)};
+  return c;
+}
+// C = A B^+, rows 0-1
+template <class T>
+MAG_HD M3<T> mul_nd_r2(const M3<T>& a, const M3<T>& b) {
+  M3<T> c;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = a(i, 0) * cconj(b(j, 0));
+      s = cfma(a(i, 1), cconj(b(j, 1)), s);
+      s = cfma(a(i, 2), cconj(b(j, 2)), s);
+      c(i, j) = s;
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) c(2, j) = {T(0), T(0)};
+  return c;
+}
+// C = A^+ B, rows 0-1
+template <class T>
+MAG_HD M3<T> mul_dn_r2(const M3<T>& a, const M3<T>& b) {
+  M3<T> c;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = cconj(a(0, i)) * b(0, j);
+      s = cfma_cj(a(1, i), b(1, j), s);
+      s = cfma_cj(a(2, i), b(2, j), s);
+      c(i, j) = s;
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) c(2, j) = {T(0), T(0)};
+  return c;
+}
+// C = A^+ B^+, rows 0-1
+template <class T>
+MAG_HD M3<T> mul_dd_r2(const M3<T>& a, const M3<T>& b) {
+  M3<T> c;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> s = cconj(a(0, i)) * cconj(b(j, 0));
+      s = s + cconj(a(1, i)) * cconj(b(j, 1));
+      s = s + cconj(a(2, i)) * cconj(b(j, 2));
+      c(i, j) = s;
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) c(2, j) = {T(0), T(0)};
+  return c;
+}
+
 // Third row of an SU(3) matrix from the first two: conj(row0 x row1).
 template <class T>
 MAG_HD void reconstruct_row2(M3<T>& m) {
   m(2, 0) = cconj(m(0, 1) * m(1, 2) - m(0, 2) * m(1, 1));
   m(2, 1) = cconj(m(0, 2) * m(1, 0) - m(0, 0) * m(1, 2));
   m(2, 2) = cconj(m(0, 0) * m(1, 1) - m(0, 1) * m(1, 0));
+}
+
+// c += m for an SU(3) matrix m given by its rows 0 and 1
+template <class T>
+MAG_HD void add_su3_r2(M3<T>& c, M3<T> m) {
+  reconstruct_row2(m);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c.e[k] = c.e[k] + m.e[k];
+}
+// c += s m for an SU(3) matrix m given by its rows 0 and 1
+template <class T>
+MAG_HD void axpy_su3_r2(M3<T>& c, T s, M3<T> m) {
+  reconstruct_row2(m);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) c.e[k] = {mfma(s, m.e[k].r, c.e[k].r), mfma(s, m.e[k].i, c.e[k].i)};
 }
 
 // Project onto SU(3): normalise row 0, orthogonalise and normalise row 1,
