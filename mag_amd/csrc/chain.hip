@@ -30,10 +30,15 @@ int traj_kernels(mag_chain c, bool use_epoch_ptr) {
   if ((r = mom_refresh(x, g, c->prec, c->P->p, c->p.seed, (uint64_t)c->traj, 1.,
                        use_epoch_ptr ? c->d_epoch : nullptr)))
     return r;
-  // H_old (field-hmc.h:812) and the thin plaquette of the old state
-  if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, rect, S_OLD_P))) return r;
+  // H_old (field-hmc.h:812) and the thin plaquette of the old state; the
+  // action sums come from the previous accept step when it left them (not
+  // while a serving batch records its fixed launch sequence)
+  c->s_used = c->s_valid && !x->rec;
+  if (!c->s_used) {
+    if ((r = O::smeared_action_sums(x, g, c->U->p, c->act.rho, c->act.n_stout, rect, S_OLD_P))) return r;
+    if ((r = O::action_sums(x, g, c->U->p, false, S_THIN_OLD))) return r;
+  }
   if ((r = kinetic_sum(x, g, c->prec, c->P->p, c->act.mag, S_OLD_K))) return r;
-  if ((r = O::action_sums(x, g, c->U->p, false, S_THIN_OLD))) return r;
   // backup, integrate (field-hmc.h:814)
   if ((r = copy_dev(x, c->Ubak->p, c->U->p, c->U->bytes))) return r;
   if (c->p.integrator == 0) r = O::fg(x, g, c->U->p, c->P->p, c->act, c->p.trajectory_length, c->p.dt, ws);
@@ -56,8 +61,15 @@ int chain_traj_kernels(mag_chain c, bool use_epoch_ptr) {
   });
 }
 
-int chain_accept(mag_chain c, const double* s, mag_traj_result* out) {
+int chain_accept(mag_chain c, const double* s_in, mag_traj_result* out) {
   mag_ctx x = c->ctx;
+  double s[S_COUNT];
+  std::memcpy(s, s_in, sizeof s);
+  if (c->s_used) {
+    s[S_OLD_P] = c->s_cache[0];
+    s[S_OLD_R] = c->s_cache[1];
+    s[S_THIN_OLD] = c->s_cache[2];
+  }
   const double S_old = action_from_sums(c->act, s[S_OLD_P], s[S_OLD_R]);
   const double S_new = action_from_sums(c->act, s[S_NEW_P], s[S_NEW_R]);
   const double h_old = s[S_OLD_K] / 2. + S_old;
@@ -89,6 +101,12 @@ int chain_accept(mag_chain c, const double* s, mag_traj_result* out) {
     int r = copy_dev(x, c->U->p, c->Ubak->p, c->U->bytes);
     if (r) return r;
   }
+  // the action sums of the links the chain now holds
+  const int k0 = acc ? S_NEW_P : S_OLD_P, k1 = acc ? S_NEW_R : S_OLD_R, k2 = acc ? S_THIN_NEW : S_THIN_OLD;
+  c->s_cache[0] = s[k0];
+  c->s_cache[1] = s[k1];
+  c->s_cache[2] = s[k2];
+  c->s_valid = std::isfinite(s[k0]) && std::isfinite(s[k1]) && std::isfinite(s[k2]);
   if (out) {
     out->h_old = h_old;
     out->h_new = h_new;
@@ -225,6 +243,7 @@ int mag_chain_run(mag_chain c, int ntraj, mag_traj_result* results) {
   if (!c || ntraj < 0) return MAG_ERR_ARG;
   int r;
   if (c->p.accept_mode != 0 && (r = chain_init_demon(c))) return r;
+  c->s_valid = false;  // the links may have been written since the last call
   for (int t = 0; t < ntraj; ++t) {
     if ((r = chain_traj_kernels(c, false))) return r;
     if ((r = fetch_scalars(c->ctx, S_COUNT))) return r;

@@ -20,6 +20,13 @@ struct mag_chain_s {
   double demon[4] = {0., 0., 0., 0.};  // S0p, S0r, Ep, Er
   double emax = 0.;
   int naccept = 0;
+  // Action sums of the current links (smeared plaquette and rectangle sums,
+  // thin plaquette sum), left by the previous trajectory's accept step: the
+  // next H_old is evaluated on exactly those links (kept on accept, restored
+  // from the backup on reject), so its sums are these values. Valid only
+  // within one mag_chain_run call (the links can be written between calls).
+  bool s_valid = false, s_used = false;
+  double s_cache[3] = {0., 0., 0.};
 };
 
 namespace mag {
