@@ -218,7 +218,7 @@ int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, mag_field Ec
  * record the average and constrained plaquettes (plaq, cons_plaq: num_iter
  * values each, may be NULL). acc_rej = {accept, reject, acceptCons,
  * rejectCons}. Order: the reference's sequential site order (one lane; the
- * same decisions as the oracle) or 16 checkerboard classes per iteration
+ * same decisions as the oracle; at most 16^4 sites) or 16 checkerboard classes per iteration
  * (mu x parity of x_mu x parity of the transverse coordinate sum, whose
  * updates share no staple; even extents). Philox stream (seed, link, iteration). */
 enum { MAG_ORDER_REFERENCE = 0, MAG_ORDER_CHECKERBOARD = 1 };

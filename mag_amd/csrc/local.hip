@@ -257,6 +257,9 @@ int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nh
   if (mag > 0)
     for (int i = 0; i < 4; ++i)
       if (g.L[i] % mag) return fail(c, MAG_ERR_GEOMETRY, "extents must be multiples of mag");
+  // the reference order is one lane walking every link: a check on small lattices
+  if (order == MAG_ORDER_REFERENCE && g.V > 65536)
+    return fail(c, MAG_ERR_UNSUPPORTED, "the reference order runs on one lane; use it up to 16^4 sites");
   const int nthr = order == MAG_ORDER_CHECKERBOARD ? g.V / 4 : 1;
   int r;
   // per-thread hit counts, in a buffer of the shape's workspace (the

@@ -49,6 +49,9 @@ def test_class_api(ctx):
     assert 0 < n < 1 and 0 < c < 1 and 0 < t < 1
     with pytest.raises(M.MagError):
         M.local_metropolis(M.Field(ctx, (8, 8, 8, 9), LINK18, F64).unit(), 5.7, 0, 1, 1, 0.5, 1)
+    with pytest.raises(M.MagError):  # the one-lane reference order is for small lattices
+        M.local_metropolis(M.Field(ctx, (16, 16, 16, 32), LINK18, F64).unit(), 5.7, 0, 1, 1, 0.5, 1,
+                           M.ORDER_REFERENCE)
 
 
 def test_strong_coupling_16(ctx):
