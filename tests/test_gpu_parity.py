@@ -142,6 +142,32 @@ def test_force_fp32_error(ctx, orc, name):
     assert err < 2e-5, err
 
 
+# Volumes of the serving stream and non-cubic shapes. The stencil tile order
+# has three paths: XCD slabs (an (x0, x1) plane of whole 256-site blocks and
+# L3 divisible by 8: 16x16x4x8), XCD-interleaved tiles (block count a multiple
+# of 8: 6^4, 4x6x8x10, 12^4, with partial last blocks in the first two) and
+# plain order (4^4). All must give the oracle's force and Hamiltonian.
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 6, 6, 6), (4, 6, 8, 10), (12, 12, 12, 12), (16, 16, 4, 8)])
+def test_force_volume_sweep(ctx, orc, L):
+    a = orc.action(**ACTIONS["iwasaki_stout3"])
+    U = orc.haar_links(L, 23)
+    ref = orc.get_force(L, U, a)
+    act = M._lib.Action(a.type, a.beta, a.c1, a.mag, a.n_stout, a.rho)
+    F = alg(ctx, L)
+    M.get_force(F, links(ctx, L, U), act)
+    assert rel(F.download(), ref) < 1e-12
+    U32 = links(ctx, L, U, F32)
+    ref32 = orc.get_force(L, U32.download(), a)
+    F32f = alg(ctx, L, prec=F32)
+    M.get_force(F32f, U32, act)
+    assert rel(F32f.download(), ref32) < 2e-5
+    P = orc.init_momentum(L, 24, 0)
+    h, parts = M.get_hamiltonian(links(ctx, L, U), alg(ctx, L, P), act)
+    h_ref, parts_ref = orc.hamiltonian(L, U, P, a)
+    assert rel(h, h_ref) < 1e-13
+    assert rel(parts, parts_ref) < 1e-13
+
+
 @pytest.mark.parametrize("mag", [0, 2])
 def test_evolve_gauge_and_momentum(ctx, orc, mag):
     U = orc.haar_links(L8, 23)
