@@ -471,12 +471,15 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
     const int opn = off_rt(g, p, nu, 1), omn = off_rt(g, p, nu, -1);
     const int xpn = p.s + opn, xmn = p.s + omn, xpmmn = xpm + omn;
     {
-      const M3<TC> D = mul_nd(a.U.template load<TC>(nu, xpm), a.U.template load<TC>(mu, xpn));
+      // D and E = D U_nu(x)^+ are products of links: formed in two rows, completed once
+      M3<TC> D = mul_nd_r2(a.U.template load_r2<TC>(nu, xpm), a.U.template load<TC>(mu, xpn));
+      reconstruct_row2(D);
       M3<TC> G = mul_hn(a.L.template load_mat<TC>(nu, xpm), D);
       mul_sub_nh(G, D, a.L.template load_mat<TC>(mu, xpn));
       const M3<TC> Un_x = a.U.template load<TC>(nu, p.s);
       mul_nd_acc(acc, G, Un_x);
-      const M3<TC> E = mul_nd(D, Un_x);
+      M3<TC> E = mul_nd_r2(D, Un_x);
+      reconstruct_row2(E);
       mul_sub_nh(acc, E, a.L.template load_mat<TC>(nu, p.s));
     }
     {
