@@ -27,13 +27,19 @@ namespace mag {
 constexpr int BLOCK = 256;
 
 // The FP32 stencil kernels need ~150-170 VGPRs. The waves per SIMD follow
-// from the VGPR budget (512 per lane): <= 168 gives 3 waves, above that 2.
+// from the VGPR budget (512 per lane): <= 168 gives 3 waves, <= 256 gives 2.
 // FP32 instances are held to 3 waves so that none falls to 2 (which cost
-// the Iwasaki top kernel 20 %); FP64 instances (compute type double) keep
-// their registers, since holding them to 168 spills hundreds of bytes.
+// the Iwasaki top kernel 20 %). FP64 stout kernels are held to 2 waves:
+// unbounded they take 256 VGPRs plus 4-20 AGPRs and run 1 wave; at 2 waves
+// they spill 20-88 bytes per lane and the backward and top kernels run 16 %
+// and 9 % faster. The FP64 thin-link Iwasaki force (two staple sums for the
+// constrained classes) would spill 536 bytes at 2 waves and keeps 1.
 template <class TC>
-constexpr int stencil_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
+constexpr int stencil_waves() { return sizeof(TC) == sizeof(float) ? 3 : 2; }
+template <class TC>
+constexpr int gforce_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
 #define MAG_STENCIL_OCC __attribute__((amdgpu_waves_per_eu(stencil_waves<TC>())))
+#define MAG_GFORCE_OCC __attribute__((amdgpu_waves_per_eu(gforce_waves<TC>())))
 
 // Blocks are dispatched round-robin over the 8 XCDs (XCD = linear block id
 // mod 8). When an (x0, x1) plane is a whole number of blocks and L3 splits
@@ -235,9 +241,9 @@ MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   force_epilogue<TS, NC, TC>(a, mu, s, Ux, f);
 }
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, TS, NC, TC>(a); }
+__global__ __launch_bounds__(BLOCK) MAG_GFORCE_OCC void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, TS, NC, TC>(a); }
 template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
+__global__ __launch_bounds__(BLOCK) MAG_GFORCE_OCC void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
 
 // ============================================================ link updates
 // evolve_gauge_field (field-hmc.h:442-482) with the constraint classes:
