@@ -53,11 +53,17 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   fa.eps = eps;
   const bool iw = act.type == MAG_IWASAKI;
   if (act.n_stout <= 0) {
+    if (iw && act.mag)
+      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, true, TS, NC, TS>,
+                    k_gauge_force_b<ACT_IWASAKI, true, TS, NC, TS>, grid_stencil(g), fa);
     if (iw)
-      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, TS, NC, TS>,
-                    k_gauge_force_b<ACT_IWASAKI, TS, NC, TS>, grid_stencil(g), fa);
-    return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, TS, NC, TS>, k_gauge_force_b<ACT_WILSON, TS, NC, TS>,
-                  grid_stencil(g), fa);
+      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, false, TS, NC, TS>,
+                    k_gauge_force_b<ACT_IWASAKI, false, TS, NC, TS>, grid_stencil(g), fa);
+    if (act.mag)
+      return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, true, TS, NC, TS>,
+                    k_gauge_force_b<ACT_WILSON, true, TS, NC, TS>, grid_stencil(g), fa);
+    return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, false, TS, NC, TS>,
+                  k_gauge_force_b<ACT_WILSON, false, TS, NC, TS>, grid_stencil(g), fa);
   }
   const int n = act.n_stout;
   if ((r = ws->need_stout(n))) return r;
