@@ -222,7 +222,7 @@ MAG_HD M3<T> ch_combine(const M3<T>& Q, const M3<T>& Q2, cpx<T> g0, cpx<T> g1, c
 // exp(iQ) (field-hmc.h:82-123).
 template <class T>
 MAG_HD M3<T> expiQ(const M3<T>& Q) {
-  const M3<T> Q2 = mul(Q, Q);
+  const M3<T> Q2 = sq_h(Q);
   T c0, c1;
   ch_invariants(Q, Q2, c0, c1);
   const CHC<T> c = ch_coef<false>(c0, c1);
@@ -233,7 +233,7 @@ MAG_HD M3<T> expiQ(const M3<T>& Q) {
 // exp(iQ) - 1 = (f0 - 1) + f1 Q + f2 Q^2, never forming 1 + (small).
 template <class T>
 MAG_HD M3<T> expiQ_m1(const M3<T>& Q) {
-  const M3<T> Q2 = mul(Q, Q);
+  const M3<T> Q2 = sq_h(Q);
   T c0, c1;
   ch_invariants(Q, Q2, c0, c1);
   const CHC<T> c = ch_coef<false>(c0, c1);
@@ -278,7 +278,7 @@ MAG_HD M3<T> exp_right(const M3<T>& U, const M3<T>& Q) {
 // Also returns exp(iQ) for the chain's Z' e^{iQ} term.
 template <class T>
 MAG_HD void ch_lambda(const M3<T>& Q, const M3<T>& X, T lam[8], M3<T>& eQ) {
-  const M3<T> Q2 = mul(Q, Q);
+  const M3<T> Q2 = sq_h(Q);
   T c0, c1;
   ch_invariants(Q, Q2, c0, c1);
   const CHC<T> c = ch_coef<true>(c0, c1);
@@ -289,9 +289,10 @@ MAG_HD void ch_lambda(const M3<T>& Q, const M3<T>& X, T lam[8], M3<T>& eQ) {
     return;
   }
   eQ = ch_combine(Q, Q2, c.f0, c.f1, c.f2);
-  const M3<T> B1 = ch_combine(Q, Q2, c.b10, c.b11, c.b12);
-  const M3<T> B2 = ch_combine(Q, Q2, c.b20, c.b21, c.b22);
-  const cpx<T> t1 = trace_mul(X, B1), t2 = trace_mul(X, B2);
+  // Tr(X B_j) with B_j = b_j0 + b_j1 Q + b_j2 Q^2, from Tr X, Tr(X Q), Tr(X Q^2)
+  const cpx<T> x0 = trace(X), x1 = trace_mul_nh(X, Q), x2 = trace_mul_nh(X, Q2);
+  const cpx<T> t1 = cfma(c.b12, x2, cfma(c.b11, x1, c.b10 * x0));
+  const cpx<T> t2 = cfma(c.b22, x2, cfma(c.b21, x1, c.b20 * x0));
   // Gamma = t1 Q + t2 Q^2 + f1 X + f2 (Q X + X Q)
   const cpx<T> g1 = c.f1, g2 = c.f2;
   M3<T> QX = mul_hn(Q, X);

@@ -145,6 +145,38 @@ MAG_HD M3<T> mul_hn(const M3<T>& h, const M3<T>& a) {
     }
   return c;
 }
+// H^2 of a Hermitian H from its upper triangle and real diagonal: the
+// diagonal is sum_k |h_ik|^2, the lower triangle the conjugate of the upper
+// (a third of the multiply-adds of a general product).
+template <class T>
+MAG_HD M3<T> sq_h(const M3<T>& h) {
+  const T d0 = h(0, 0).r, d1 = h(1, 1).r, d2 = h(2, 2).r;
+  const cpx<T> u01 = h(0, 1), u02 = h(0, 2), u12 = h(1, 2);
+  const T n01 = mfma(u01.r, u01.r, u01.i * u01.i);
+  const T n02 = mfma(u02.r, u02.r, u02.i * u02.i);
+  const T n12 = mfma(u12.r, u12.r, u12.i * u12.i);
+  M3<T> r;
+  r(0, 0) = {mfma(d0, d0, n01 + n02), T(0)};
+  r(1, 1) = {mfma(d1, d1, n01 + n12), T(0)};
+  r(2, 2) = {mfma(d2, d2, n02 + n12), T(0)};
+  r(0, 1) = rfma(d0 + d1, u01, u02 * cconj(u12));
+  r(0, 2) = rfma(d0 + d2, u02, u01 * u12);
+  r(1, 2) = rfma(d1 + d2, u12, cconj(u01) * u02);
+  r(1, 0) = cconj(r(0, 1));
+  r(2, 0) = cconj(r(0, 2));
+  r(2, 1) = cconj(r(1, 2));
+  return r;
+}
+// Tr(A H) for a Hermitian H (real diagonal)
+template <class T>
+MAG_HD cpx<T> trace_mul_nh(const M3<T>& a, const M3<T>& h) {
+  cpx<T> s = {T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s = (k == i) ? rfma(h(i, i).r, a(i, i), s) : cfma(a(i, k), h(k, i), s);
+  return s;
+}
 // C += A H
 template <class T>
 MAG_HD void mul_acc_nh(M3<T>& c, const M3<T>& a, const M3<T>& h) {

@@ -22,8 +22,11 @@ def _totals(demons):
     return np.array([d.download().sum() for d in demons])
 
 
-def test_matches_oracle(ctx, orc):
-    L = (8, 8, 8, 8)
+# 8^4: 32 links per class, whole wavefronts of 8 links; 4x8x4x12: 12 links
+# per class, the second wavefront half filled (its surplus lanes repeat the
+# last link and must write nothing)
+@pytest.mark.parametrize("L", [(8, 8, 8, 8), (4, 8, 4, 12)])
+def test_matches_oracle(ctx, orc, L):
     U0 = orc.haar_links(L, 151)
     f = M.Field(ctx, L, LINK18, F64).upload(U0)
     d, (acc, rej) = M.demon_checkerboard(f, 2, 1.5, 152, _demons(ctx, L))
