@@ -463,33 +463,40 @@ struct DWalk {
 
 template <int MU>
 MAG_D void cforce_link(const MultiArgs& a, const int* x0, int sc) {
-  // -i TA of Omega = stp U2^+ and of its derivative, summed over the 12
-  // rectangle loops (get_DQ) as algebra components; the nu loop stays rolled
-  // (one nu's loops live at a time).
-  D q[8] = {0., 0., 0., 0., 0., 0., 0., 0.}, dq[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-  auto add_loop = [&](const M3<D>& lm, const M3<D>& ld) {
-    D t[8], dt[8];
-    force_alg_from(lm, t);
-    force_alg_from(ld, dt);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      q[k] += t[k];
-      dq[k] += dt[k];
-    }
-  };
+  // -i TA of Omega = stp U2^+ and of its derivative over the 6 rectangle
+  // loops (get_DQ). Every loop is a 2x1 staple from x0 to x0 + 2 mu closed by
+  // the same two links back, and the (product, derivative) step is linear in
+  // the pair, so the staples and their derivatives are summed first and the
+  // closing links applied once. The nu loop stays rolled (one nu's paths
+  // live at a time).
+  M3<D> S = zero3<D>(), dS = zero3<D>();
 #pragma unroll 1
   for (int j = 0; j < 3; ++j) {
     const int nu = j < MU ? j : j + 1;
     {
       DWalk w(a, x0);
-      w.fwd(nu); w.fwd(MU); w.fwd(MU); w.bwd(nu); w.bwd(MU); w.bwd(MU);
-      add_loop(w.m, w.d);
+      w.fwd(nu); w.fwd(MU); w.fwd(MU); w.bwd(nu);
+      add_to(S, w.m);
+      add_to(dS, w.d);
     }
     {
       DWalk w(a, x0);
-      w.bwd(nu); w.fwd(MU); w.fwd(MU); w.fwd(nu); w.bwd(MU); w.bwd(MU);
-      add_loop(w.m, w.d);
+      w.bwd(nu); w.fwd(MU); w.fwd(MU); w.fwd(nu);
+      add_to(S, w.m);
+      add_to(dS, w.d);
     }
+  }
+  D q[8], dq[8];
+  {
+    DWalk w(a, x0);
+    shift<MU>(a.gf, w.y, 2);
+    w.m = S;
+    w.d = dS;
+    w.fresh = false;
+    w.bwd(MU);
+    w.bwd(MU);
+    force_alg_from(w.m, q);
+    force_alg_from(w.d, dq);
   }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
