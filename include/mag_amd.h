@@ -122,9 +122,10 @@ int mag_init_momentum(mag_field P, uint64_t seed, uint64_t traj, double m);
 /* reunitarize (field-matrix.h:83-100) */
 int mag_reunitarize(mag_field U, double* max_dev);
 /* expiQ (field-hmc.h:82-123), batch of n 3x3 matrices [n][18]; prec selects
- * the arithmetic of the 3x3 algebra */
+ * the arithmetic of the 3x3 algebra. Q is Hermitian traceless, as in the
+ * reference: Q^2 is formed from its upper triangle and real diagonal. */
 int mag_expiQ_batch(mag_ctx ctx, const double* Q, double* out, long n, int prec);
-/* compute_Lambda (field-multi.h:180-269), batch of n */
+/* compute_Lambda (field-multi.h:180-269), batch of n; Q Hermitian traceless */
 int mag_compute_Lambda_batch(mag_ctx ctx, const double* Q, const double* SigmaP, const double* U, double* out,
                              long n, int prec);
 
