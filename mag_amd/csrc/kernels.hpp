@@ -32,14 +32,14 @@ constexpr int BLOCK = 256;
 // the Iwasaki top kernel 20 %). FP64 stout kernels are held to 2 waves:
 // unbounded they take 256 VGPRs plus 4-20 AGPRs and run 1 wave; at 2 waves
 // they spill 20-88 bytes per lane and the backward and top kernels run 16 %
-// and 9 % faster. The FP64 thin-link Iwasaki force (two staple sums for the
-// constrained classes) would spill 536 bytes at 2 waves and keeps 1.
+// and 9 % faster. The FP64 thin-link force of constrained runs (two staple
+// sums for the constraint classes) would spill 536 bytes at 2 waves and keeps 1.
 template <class TC>
 constexpr int stencil_waves() { return sizeof(TC) == sizeof(float) ? 3 : 2; }
-template <class TC>
-constexpr int gforce_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
+template <class TC, bool CONS>
+constexpr int gforce_waves() { return sizeof(TC) == sizeof(float) ? 3 : (CONS ? 1 : 2); }
 #define MAG_STENCIL_OCC __attribute__((amdgpu_waves_per_eu(stencil_waves<TC>())))
-#define MAG_GFORCE_OCC __attribute__((amdgpu_waves_per_eu(gforce_waves<TC>())))
+#define MAG_GFORCE_OCC __attribute__((amdgpu_waves_per_eu(gforce_waves<TC, CONS>())))
 
 // Blocks are dispatched round-robin over the 8 XCDs (XCD = linear block id
 // mod 8). When an (x0, x1) plane is a whole number of blocks and L3 splits
@@ -209,7 +209,8 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 // free links, W = U(x+mu) S(x+mu)^+ - S(x)^+ U(x) for constraint classes 1 and
 // 10, and F = 0 for class 100. Stored as F_a = -(beta/3) Im Tr(T_a W).
 // CONS = false (mag = 0) compiles the free-link form alone (one staple sum,
-// no class loop): 5 % faster for the FP64 Iwasaki force.
+// no class loop, U(x) loaded after it); its FP64 instance is held to 2 waves
+// per SIMD (88 bytes of spill), 13 % faster than the class-loop kernel at 1.
 template <int ACT, bool CONS, class TS, int NC, class TC>
 MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   int mu, sblk;
@@ -219,20 +220,27 @@ MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
   const TC c0 = TC(1) - TC(8) * c1;
   const Site p = site_of(a.g, s);
-  const int cls = CONS ? link_class_rt(p, mu, a.mag) : 0;
-  const M3<TC> Ux = a.U.template load<TC>(mu, s);
-  M3<TC> W = zero3<TC>();
-  if (cls != 100) {
-    // class 0: one staple at x; classes 1, 10: staples at x and x+mu
-    const int nsite = (cls == 0) ? 1 : 2;
-    const int ys = s + off_rt(a.g, p, mu, 1);
+  M3<TC> Ux, W = zero3<TC>();
+  if (!CONS) {
+    // U(x) is loaded after the staple sum, so it is not live across it
+    const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, p, mu, c0, c1);
+    Ux = a.U.template load<TC>(mu, s);
+    W = mul_nd(Ux, S);
+  } else {
+    const int cls = link_class_rt(p, mu, a.mag);
+    Ux = a.U.template load<TC>(mu, s);
+    if (cls != 100) {
+      // class 0: one staple at x; classes 1, 10: staples at x and x+mu
+      const int nsite = (cls == 0) ? 1 : 2;
+      const int ys = s + off_rt(a.g, p, mu, 1);
 #pragma unroll 1
-    for (int k = 0; k < nsite; ++k) {
-      const Site q = (k == 0) ? p : site_of(a.g, ys);
-      const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, q, mu, c0, c1);
-      if (cls == 0) W = mul_nd(Ux, S);
-      else if (k == 0) W = sub(W, mul_dn(S, Ux));
-      else mul_nd_acc(W, a.U.template load<TC>(mu, ys), S);
+      for (int k = 0; k < nsite; ++k) {
+        const Site q = (k == 0) ? p : site_of(a.g, ys);
+        const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, q, mu, c0, c1);
+        if (cls == 0) W = mul_nd(Ux, S);
+        else if (k == 0) W = sub(W, mul_dn(S, Ux));
+        else mul_nd_acc(W, a.U.template load<TC>(mu, ys), S);
+      }
     }
   }
   TC f[8];
