@@ -191,17 +191,17 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
     a.F.store(mu, s, f);
   } else if (a.out == FOUT_PADD) {
     TC p[8];
-    a.P.load(mu, s, p);
+    a.P.load_nt(mu, s, p);
     const TC e = TC(a.eps);
 #pragma unroll
     for (int k = 0; k < 8; ++k) p[k] = mfma(e, f[k], p[k]);
-    a.P.store(mu, s, p);
+    a.P.store_nt(mu, s, p);
   } else {
     const TC e = TC(a.eps);
 #pragma unroll
     for (int k = 0; k < 8; ++k) f[k] *= e;
     const M3<TC> Q = alg_to_mat(f);
-    a.Uaux.store(mu, s, exp_left(Q, Ux));
+    a.Uaux.store_nt(mu, s, exp_left(Q, Ux));
   }
 }
 
@@ -277,10 +277,10 @@ MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
   const Site p = site_of(a.g, s);
   const int cls = link_class_rt(p, mu, a.mag);
   const TC e = TC(a.eps);
-  M3<TC> U = a.Uin.template load<TC>(mu, s);
+  M3<TC> U = a.Uin.template load_nt<TC>(mu, s);
   TC h[8];
   if (cls == 0 || cls == 1 || cls == 10) {
-    a.P.load(mu, s, h);
+    a.P.load_nt(mu, s, h);
     const TC sg = (cls == 0) ? e : -e;
 #pragma unroll
     for (int k = 0; k < 8; ++k) h[k] *= sg;
@@ -293,7 +293,7 @@ MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
     for (int k = 0; k < 8; ++k) h[k] *= e;
     U = exp_left(alg_to_mat(h), U);
   }
-  a.Uout.store(mu, s, U);
+  a.Uout.store_nt(mu, s, U);
 }
 template <class TS, int NC, class TC>
 __global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) { k_link_update_body<TS, NC, TC>(a); }
@@ -354,10 +354,10 @@ MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) q[k] *= rho;
   const M3<TC> Q = alg_to_mat(q);
-  a.Unext.store(mu, s, exp_left(Q, Ux));
+  a.Unext.store_nt(mu, s, exp_left(Q, Ux));
   if (a.store_qc) {
-    a.Q.store(mu, s, q);
-    a.C.store(mu, s, scale(S, rho));
+    a.Q.store_nt(mu, s, q);
+    a.C.store_nt(mu, s, scale(S, rho));
   }
 }
 template <class TSI, int NCI, class TSO, int NCO, class TC>
@@ -372,20 +372,20 @@ __global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd_b(const Sto
 template <class TS, int NC, class TC>
 MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS, NC>& U, const AlgView<TS>& Qv,
                              const MatView<TS>& Cv, const AlgView<TS>& Lout, const MatView<TS>& Wout) {
-  const M3<TC> Uk = U.template load<TC>(mu, s);
-  const M3<TC> Q = Qv.template load_mat<TC>(mu, s);
+  const M3<TC> Uk = U.template load_nt<TC>(mu, s);
+  const M3<TC> Q = Qv.template load_mat_nt<TC>(mu, s);
   const M3<TC> X = mul(Uk, Zp);
   TC lam[8];
   M3<TC> eQ;
   ch_lambda(Q, X, lam, eQ);
-  Lout.store(mu, s, lam);
+  Lout.store_nt(mu, s, lam);
   const M3<TC> Lm = alg_to_mat(lam);
   M3<TC> W = mul(Zp, eQ);
   // + i C^+ Lambda
-  const M3<TC> CL = mul_dh(Cv.template load<TC>(mu, s), Lm);
+  const M3<TC> CL = mul_dh(Cv.template load_nt<TC>(mu, s), Lm);
 #pragma unroll
   for (int k = 0; k < 9; ++k) W.e[k] = W.e[k] + cpx<TC>{-CL.e[k].i, CL.e[k].r};
-  Wout.store(mu, s, W);
+  Wout.store_nt(mu, s, W);
 }
 
 // Top of the chain: Z_n = -(beta/3) S(U_n)^+ (derivative of the gauge action
@@ -512,7 +512,7 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
     }
   }
   // Z = W + i rho acc
-  M3<TC> Z = a.W.template load<TC>(mu, p.s);
+  M3<TC> Z = a.W.template load_nt<TC>(mu, p.s);
   const TC rho = TC(a.rho);
 #pragma unroll
   for (int k = 0; k < 9; ++k) Z.e[k] = {mfma(-rho, acc.e[k].i, Z.e[k].r), mfma(rho, acc.e[k].r, Z.e[k].i)};

@@ -144,6 +144,30 @@ struct LinkView {
       at(plane(p, (size_t)(mu * NC + 2 * k + 1), V), u) = TS(m.e[k].i);
     }
   }
+  // Non-temporal forms for data a kernel touches once, at its own link
+  // (read or written by this thread only): they do not displace the
+  // neighbour links that the stencil reuses from L2.
+  template <class TC>
+  MAG_D M3<TC> load_nt(int mu, int s) const {
+    const unsigned u = (unsigned)s;
+    M3<TC> m;
+#pragma unroll
+    for (int k = 0; k < NC / 2; ++k) {
+      m.e[k].r = TC(__builtin_nontemporal_load(&at(plane(p, (size_t)(mu * NC + 2 * k), V), u)));
+      m.e[k].i = TC(__builtin_nontemporal_load(&at(plane(p, (size_t)(mu * NC + 2 * k + 1), V), u)));
+    }
+    if (NC == 12) reconstruct_row2(m);
+    return m;
+  }
+  template <class TC>
+  MAG_D void store_nt(int mu, int s, const M3<TC>& m) const {
+    const unsigned u = (unsigned)s;
+#pragma unroll
+    for (int k = 0; k < NC / 2; ++k) {
+      __builtin_nontemporal_store(TS(m.e[k].r), &at(plane(p, (size_t)(mu * NC + 2 * k), V), u));
+      __builtin_nontemporal_store(TS(m.e[k].i), &at(plane(p, (size_t)(mu * NC + 2 * k + 1), V), u));
+    }
+  }
 };
 
 template <class TS>
@@ -166,6 +190,24 @@ struct AlgView {
   MAG_D M3<TC> load_mat(int mu, int s) const {
     TC h[8];
     load(mu, s, h);
+    return alg_to_mat(h);
+  }
+  template <class TC>
+  MAG_D void load_nt(int mu, int s, TC h[8]) const {
+    const unsigned u = (unsigned)s;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) h[a] = TC(__builtin_nontemporal_load(&at(plane(p, (size_t)(mu * 8 + a), V), u)));
+  }
+  template <class TC>
+  MAG_D void store_nt(int mu, int s, const TC h[8]) const {
+    const unsigned u = (unsigned)s;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) __builtin_nontemporal_store(TS(h[a]), &at(plane(p, (size_t)(mu * 8 + a), V), u));
+  }
+  template <class TC>
+  MAG_D M3<TC> load_mat_nt(int mu, int s) const {
+    TC h[8];
+    load_nt(mu, s, h);
     return alg_to_mat(h);
   }
 };
