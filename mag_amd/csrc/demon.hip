@@ -306,8 +306,8 @@ __global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
   X[1] = dag(X[1]);
   double E[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) E[k] = A.E[k][(size_t)MU * A.g.V + s];
-  M3<double> v = A.U.template load<double>(MU, s);
+  for (int k = 0; k < 4; ++k) E[k] = __builtin_nontemporal_load(&A.E[k][(size_t)MU * A.g.V + s]);
+  M3<double> v = A.U.template load_nt<double>(MU, s);
   const uint32_t link = (uint32_t)s * 4u + (uint32_t)MU;
   double acc = 0., rej = 0.;
 #pragma unroll 1
@@ -346,9 +346,9 @@ __global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
       rej += 1.;
     }
   }
-  A.U.store(MU, s, v);
+  A.U.store_nt(MU, s, v);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) A.E[k][(size_t)MU * A.g.V + s] = E[k];
+  for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(E[k], &A.E[k][(size_t)MU * A.g.V + s]);
   A.cnt[(size_t)t * 2] += acc;
   A.cnt[(size_t)t * 2 + 1] += rej;
 }

@@ -140,7 +140,7 @@ MAG_D void update_link(const LocalArgs& A, const int* x, int mu, double* c) {
     shift_rt(A.g, y, mu, 1);
     const int sy = lin(A.g, y);
     const M3<D> s1 = staple(A, x, mu), s2 = staple(A, y, mu);
-    M3<D> U1 = A.U.template load<D>(mu, s), U2 = A.U.template load<D>(mu, sy);
+    M3<D> U1 = A.U.template load_nt<D>(mu, s), U2 = A.U.template load_nt<D>(mu, sy);
     // the action of the kept links carries over from hit to hit
     double oldS = -A.fbeta * retrace_mul(U1, s1);
     oldS += -A.fbeta * retrace_mul(U2, s2);
@@ -160,11 +160,11 @@ MAG_D void update_link(const LocalArgs& A, const int* x, int mu, double* c) {
         c[3] += 1.;
       }
     }
-    A.U.store(mu, s, U1);
-    A.U.store(mu, sy, U2);
+    A.U.store_nt(mu, s, U1);
+    A.U.store_nt(mu, sy, U2);
   } else {
     const M3<D> sg = staple(A, x, mu);
-    M3<D> U = A.U.template load<D>(mu, s);
+    M3<D> U = A.U.template load_nt<D>(mu, s);
     double oldS = -A.fbeta * retrace_mul(U, sg);
 #pragma unroll 1
     for (int h = 0; h < A.nhits; ++h) {
@@ -180,7 +180,7 @@ MAG_D void update_link(const LocalArgs& A, const int* x, int mu, double* c) {
         c[1] += 1.;
       }
     }
-    A.U.store(mu, s, U);
+    A.U.store_nt(mu, s, U);
   }
 }
 
