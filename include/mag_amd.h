@@ -177,7 +177,13 @@ int mag_chain_destroy(mag_chain c);
 int mag_chain_set_links_aos(mag_chain c, const double* host);
 int mag_chain_get_links_aos(mag_chain c, double* host);
 int mag_chain_haar_links(mag_chain c, uint64_t seed);
-int mag_chain_links(mag_chain c, mag_field* U);  /* borrowed view of the chain's links */
+/* Borrowed view of the chain's links. The demon window centre S0 and the
+ * demon energies (field-demon.h:152-153) are taken from the links at the
+ * first mag_chain_run or mag_chain_demon_state call and kept; set_links_aos
+ * and haar_links drop them. After writing the links through this handle,
+ * call mag_chain_reset_demon so the window is re-centred on the new links. */
+int mag_chain_links(mag_chain c, mag_field* U);
+int mag_chain_reset_demon(mag_chain c);
 /* Run ntraj trajectories; results[i] for each (may be NULL). Trajectory
  * indices continue across calls (they key the momentum and die-roll streams). */
 int mag_chain_run(mag_chain c, int ntraj, mag_traj_result* results);
@@ -315,7 +321,11 @@ int mag_multi_run(mag_multi m, int ntraj, mag_multi_result* results);
  * (x fastest), mu 0..3 per site, row-major complex IEEE64 big-endian; 18
  * reals per link (DATATYPE 4D_SU3_GAUGE_3x3) or the first two rows, 12 reals
  * (4D_SU3_GAUGE, third row rebuilt by reunitarize on import). CHECKSUM is
- * the 32-bit sum of the data taken as 32-bit words. */
+ * the 32-bit sum of the data taken as 32-bit words of the host-order doubles
+ * (what the reference's import checks, field-matrix.h:684, 697). The
+ * reference's export of 4D_SU3_GAUGE sums the byte-swapped words instead
+ * (field-matrix.h:473-477); the reader accepts that sum for 4D_SU3_GAUGE and
+ * sets checksum_swapped. */
 typedef struct {
   int dims[4];
   int skip_third;            /* 1: 4D_SU3_GAUGE (two rows); 0: 4D_SU3_GAUGE_3x3 */
@@ -325,7 +335,8 @@ typedef struct {
   int sequence_number;       /* SEQUENCE_NUMBER */
   char ensemble_label[128];  /* ENSEMBLE_LABEL */
   uint32_t checksum;         /* CHECKSUM */
-  uint32_t crc32;            /* CRC32HASH: CRC-32 (IEEE) of the data section as stored */
+  uint32_t crc32;            /* CRC32HASH: CRC-32 (IEEE) of the data in host (little-endian) byte order */
+  int checksum_swapped;      /* read: 1 if CHECKSUM matched the sum over the byte-swapped words */
 } mag_nersc_header;
 
 /* Host-side file I/O of links in the qlat AoS layout [site][mu][3][3][re,im]

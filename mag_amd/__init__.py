@@ -436,6 +436,11 @@ class Chain:
         kind = LINK12 if self.params.precision == F32 else LINK18
         return Field(self.ctx, self.dims, kind, self.params.precision, _handle=h)
 
+    def reset_demon(self):
+        """Re-centre the demon window on the links the chain holds now (after
+        writing them through links())."""
+        _check(_lib.lib().mag_chain_reset_demon(self._h), self.ctx.handle)
+
     def run(self, ntraj):
         res = (TrajResult * max(ntraj, 1))()
         _check(_lib.lib().mag_chain_run(self._h, ntraj, res), self.ctx.handle)
@@ -501,7 +506,8 @@ class Arg_export:
 def _header_dict(h):
     return {"dims": tuple(h.dims), "skip_third": bool(h.skip_third), "link_trace": h.link_trace,
             "plaquette": h.plaquette, "beta": h.beta, "sequence_number": h.sequence_number,
-            "ensemble_label": h.ensemble_label.decode(), "checksum": h.checksum, "crc32": h.crc32}
+            "ensemble_label": h.ensemble_label.decode(), "checksum": h.checksum, "crc32": h.crc32,
+            "checksum_swapped": bool(h.checksum_swapped)}
 
 
 def export_config_nersc(U, path, arg=None, does_skip_third=False):

@@ -42,7 +42,7 @@ class TrajResult(C.Structure):
 class NerscHeader(C.Structure):
     _fields_ = [("dims", C.c_int * 4), ("skip_third", C.c_int), ("link_trace", C.c_double), ("plaquette", C.c_double),
                 ("beta", C.c_double), ("sequence_number", C.c_int), ("ensemble_label", C.c_char * 128),
-                ("checksum", C.c_uint32), ("crc32", C.c_uint32)]
+                ("checksum", C.c_uint32), ("crc32", C.c_uint32), ("checksum_swapped", C.c_int)]
 
 
 class ServeRequest(C.Structure):
@@ -110,6 +110,7 @@ SIGNATURES = {
     "mag_chain_get_links_aos": [_P, _D],
     "mag_chain_haar_links": [_P, C.c_uint64],
     "mag_chain_links": [_P, C.POINTER(_P)],
+    "mag_chain_reset_demon": [_P],
     "mag_chain_run": [_P, C.c_int, C.POINTER(TrajResult)],
     "mag_chain_reversibility": [_P, _D, _D],
     "mag_chain_demon_state": [_P, _D],

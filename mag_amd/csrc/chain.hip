@@ -239,6 +239,16 @@ int mag_chain_links(mag_chain c, mag_field* U) {
   return MAG_OK;
 }
 
+// Links written through the borrowed handle: drop the demon window centre
+// (and demon energies) so the next run or demon_state call takes them from
+// the links the chain now holds.
+int mag_chain_reset_demon(mag_chain c) {
+  if (!c) return MAG_ERR_ARG;
+  c->demon_init = false;
+  c->s_valid = false;
+  return MAG_OK;
+}
+
 int mag_chain_run(mag_chain c, int ntraj, mag_traj_result* results) {
   if (!c || ntraj < 0) return MAG_ERR_ARG;
   int r;

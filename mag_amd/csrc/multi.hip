@@ -1028,7 +1028,7 @@ int mag_multi_run(mag_multi mc, int ntraj, mag_multi_result* results) {
     const double h_old = po[0] + po[1] + po[2], h_new = pn[0] + pn[1] + pn[2];
     const double die = host_uniform(mc->p.seed, RNG_ACCEPT, (uint64_t)i);
     bool acc = die < std::exp(h_old - h_new);
-    const bool bad = std::isnan(h_old) || std::isnan(h_new);
+    const bool bad = !std::isfinite(h_old) || !std::isfinite(h_new);
     if (i < mc->p.num_forced_accept && !bad) acc = true;
     if (bad) acc = false;
     // accept: the new state becomes the backup; reject: restore both fields

@@ -15,12 +15,20 @@
 // first 2 (12 reals, 4D_SU3_GAUGE). Floating header values are printed as a
 // C++ ostream prints them: 12 significant digits in the two-row format (the
 // reference sets precision(12) there, field-matrix.h:490), 6 otherwise.
-// CHECKSUM is the sum mod 2^32 of the data taken as 32-bit words, i.e. of
-// the high and low halves of every double (the order of the halves does not
-// change the sum). CRC32HASH here is the IEEE CRC-32 (zlib's crc32) of the
-// data section as stored; the reference takes it from qlat's
-// field_hash_crc32, which is not part of the reference sources, and its
-// import does not check it.
+// CHECKSUM is the sum mod 2^32 of the data taken as 32-bit words of the
+// host (little-endian) doubles, i.e. of the high and low halves of every
+// double (the order of the halves does not change the sum). That is what the
+// reference's import checks for both DATATYPEs (field-matrix.h:684, 697) and
+// what its export writes for 4D_SU3_GAUGE_3x3 (field-matrix.h:527-529, before
+// the byte swap). For 4D_SU3_GAUGE its export sums after the byte swap
+// (field-matrix.h:473-477), i.e. the stored big-endian bytes taken as
+// little-endian words, so its own import refuses those files. The writer here
+// always writes the host-order sum (readable by the reference's import); the
+// reader accepts either sum for 4D_SU3_GAUGE and reports which one matched.
+// CRC32HASH here is the IEEE CRC-32 (zlib's crc32) of the data in host byte
+// order, the bytes the reference hands to qlat's field_hash_crc32 before its
+// byte swap (field-matrix.h:470, 523); field_hash_crc32 itself is not part of
+// the reference sources and the reference's import does not check it.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -66,6 +74,18 @@ uint32_t checksum32(const double* d, size_t n) {
   return s;
 }
 
+// The same sum over the byte-swapped doubles (the stored big-endian bytes
+// taken as little-endian 32-bit words): each half byte-reversed.
+uint32_t checksum32_swapped(const double* d, size_t n) {
+  uint32_t s = 0;
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t w;
+    std::memcpy(&w, d + i, 8);
+    s += __builtin_bswap32((uint32_t)(w & 0xFFFFFFFFu)) + __builtin_bswap32((uint32_t)(w >> 32));
+  }
+  return s;
+}
+
 std::string gfmt(double v, int prec) {
   char b[64];
   std::snprintf(b, sizeof b, "%.*g", prec, v);
@@ -101,8 +121,9 @@ int mag_nersc_write(const char* path, const double* aos, mag_nersc_header* h) {
   std::vector<double> data((size_t)nlink * per);
   for (long l = 0; l < nlink; ++l) std::memcpy(&data[(size_t)l * per], aos + (size_t)l * 18, per * sizeof(double));
   h->checksum = checksum32(data.data(), data.size());
-  swap64((unsigned char*)data.data(), data.size());
+  h->checksum_swapped = 0;
   h->crc32 = crc32_ieee((const unsigned char*)data.data(), data.size() * 8);
+  swap64((unsigned char*)data.data(), data.size());
   const int prec = h->skip_third ? 12 : 6;
   char hex[32];
   std::string o = "BEGIN_HEADER\nHDR_VERSION = 1.0\n";
@@ -177,7 +198,12 @@ int mag_nersc_read(const char* path, double* aos, mag_nersc_header* h) {
   std::fclose(f);
   if (!full) return MAG_ERR_IO;
   swap64((unsigned char*)data.data(), data.size());
-  if (checksum32(data.data(), data.size()) != h->checksum) return MAG_ERR_CHECKSUM;
+  h->checksum_swapped = 0;
+  if (checksum32(data.data(), data.size()) != h->checksum) {
+    // 4D_SU3_GAUGE as the reference's export writes it (sum after the swap)
+    if (!h->skip_third || checksum32_swapped(data.data(), data.size()) != h->checksum) return MAG_ERR_CHECKSUM;
+    h->checksum_swapped = 1;
+  }
   for (long l = 0; l < nlink; ++l) {
     double* d = aos + (size_t)l * 18;
     std::memcpy(d, &data[(size_t)l * per], per * sizeof(double));

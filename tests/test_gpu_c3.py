@@ -26,6 +26,7 @@ def test_c3_32x4_100_trajectories_with_reversibility(ctx):
     ch = M.Chain(ctx, L32, M.Arg_chmc(num_forced_accept_step=0, **kw), seed=34, precision=F32, accept_mode=2,
                  demon_emax=-0.005)
     ch.links().copy_from(therm.links())
+    ch.reset_demon()
     ch.demon_state()
     res, rev = [], []
     for block in range(10):
