@@ -9,10 +9,19 @@ FP64 Hamiltonian / accept. One step = one full trajectory (momentum refresh,
 H_old, 12 FG steps = 48 force evaluations and 49 link updates, H_new, accept).
 Inputs: Haar-random links from a fixed seed (synthetic), resident in HBM.
 
+Run order: thermalisation (untimed, its own fixed rule, independent of the
+flags), then the measured FP32-MD chain (--warmup untimed trajectories, then
+--steps timed ones), the FP64 chain from the same links and seed (same
+momenta, so the two chains' trajectories pair one by one while their accept
+decisions agree), an FP64 reversibility round trip, the serving stream, and
+the FP64 CPU oracle at the same lattice (force and plaquette errors of the
+FP32 and FP64 device paths on the thermalised links, and the CPU baseline).
+
 Prints one JSON line: traj/s (value), link-updates/s, force GB/s, the FP64
-path's traj/s on the same workload, the roofline of the dominant kernel
-measured with HIP events on the library's stream, and the CPU oracle timed on
-a bounded sample of the same workload on the host cores.
+path's traj/s on the same workload, both acceptance rates with the paired
+Metropolis-probability difference, the roofline of the dominant kernel
+measured with HIP events on the library's stream, the serving stream's
+latency and throughput, and the CPU oracle timed on the host cores.
 """
 import argparse
 import dataclasses
@@ -61,17 +70,20 @@ ACTIONS = {"iwasaki": (1, 2.6, -0.331), "dbw2": (1, 1.04, -1.4069), "wilson": (0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=30, help="forced-accept thermalisation trajectories (untimed)")
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5,
+                   help="untimed trajectories of the measured chains before the timed ones (thermalisation "
+                        "is separate: see THERM_*)")
     p.add_argument("--L", type=int, default=32)
     p.add_argument("--n-md", type=int, default=12)
     p.add_argument("--n-stout", type=int, default=3)
     p.add_argument("--fp64-steps", type=int, default=-1,
                    help="trajectories of the FP64 comparison chain (default: as many as --steps; 0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--serve-chains", type=int, default=64,
-                   help="chains of the serving sample (0: skip); the full stream has 512")
-    p.add_argument("--serve-max-traj", type=int, default=10, help="trajectory cap per chain in the sample")
+    p.add_argument("--serve-chains", type=int, default=512,
+                   help="chains of the serving stream (0: skip); the full stream has 512")
+    p.add_argument("--serve-max-traj", type=int, default=50, help="trajectories per chain drawn from 5..this")
+    p.add_argument("--therm-min", type=int, default=40, help="fewest thermalisation trajectories")
     p.add_argument("--seed", type=int, default=32)
     p.add_argument("--action", choices=sorted(ACTIONS), default="iwasaki",
                    help="gauge action (dbw2 with --L 64 --n-md 16 --wloop is the 64^4 DBW2 workload)")
@@ -86,48 +98,92 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(L_full, n_md, n_stout):
-    """The FP64 CPU oracle on a bounded sample of the same workload: on a 16^4
-    lattice with the same action, one warm-up force then the median of 3 force
-    evaluations, 3 link updates and 1 Hamiltonian, scaled by volume to one
-    L^4 trajectory with the FG schedule (4 n forces, 4 n + 1 link updates,
-    2 Hamiltonians)."""
+def oracle_leg(M, ctx, L, arg, act_kind, beta, c1, n_stout, n_md, therm):
+    """The FP64 CPU oracle at the measured lattice, on the thermalised links.
+
+    1. Errors of the device forces against the oracle's FP64 force on the same
+       links: the FP32 12-real links widened to FP64 and reunitarized in FP64
+       (the device kernels form staple paths on two rows and rebuild the third,
+       exact for links in SU(3); the widened FP32 links are off the group by
+       ~1e-7), then rounded back to FP32 for the FP32-MD force. Reported as
+       max |F_dev - F_ref| / max |F_ref| and as the RMS relative error, for
+       the FP32-MD and the FP64 force; and the FP64 plaquette of both device
+       fields against the oracle's.
+    2. The CPU baseline (BASELINE.md §2 schedule): the oracle's force (the
+       force of step 1 is its warm-up; then the median of 3), link update
+       (median of 3) and Hamiltonian (median of 3) at the same lattice, composed
+       into one trajectory with the FG schedule (4n forces, 4n + 1 link
+       updates, 2 Hamiltonians)."""
     import statistics
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
-    Ls = (16, 16, 16, 16)
-    a = O.action(1, 2.6, -0.331, n_stout=n_stout, rho=0.1)
-    U, P = O.haar_links(Ls, 7), O.init_momentum(Ls, 7, 0)
-    O.get_force(Ls, U, a)
+    dims = (L,) * 4
+    a = O.action(act_kind, beta, c1, n_stout=n_stout, rho=0.1)
+    U64 = M.Field(ctx, dims, M.LINK18, M.F64).copy_from(therm.links())
+    M.reunitarize(U64)
+    U32 = M.Field(ctx, dims, M.LINK12, M.F32).copy_from(U64)
+    F32 = M.Field(ctx, dims, M.ALG8, M.F32)
+    F64 = M.Field(ctx, dims, M.ALG8, M.F64)
+    M.get_force(F32, U32, arg)
+    M.get_force(F64, U64, arg)
+    ctx.synchronize()
+    U = U64.download()
+    f32, f64 = F32.download(), F64.download()
+    p32, p64 = M.avg_plaquette(U32), M.avg_plaquette(U64)
+    del F32, F64, U64, U32
+    print(f"bench: oracle force at {L}^4 ({O.num_threads()} threads)", file=sys.stderr, flush=True)
 
     def timed(fn):
         t = time.perf_counter()
-        fn()
-        return time.perf_counter() - t
+        out = fn()
+        return time.perf_counter() - t, out
 
-    tf = statistics.median(timed(lambda: O.get_force(Ls, U, a)) for _ in range(3))
-    tu = statistics.median(timed(lambda: O.evolve_gauge_field(Ls, U, P, 0.01, 0)) for _ in range(3))
-    th = timed(lambda: O.hamiltonian(Ls, U, P, a))
-    scale = (L_full ** 4) / (16 ** 4)
-    t_traj = scale * (4 * n_md * tf + (4 * n_md + 1) * tu + 2 * th)
-    return {"value": round(1.0 / t_traj, 8), "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"FP64 oracle on 16^4, same action: force {tf:.2f} s, link update {tu:.3f} s, "
-                      f"Hamiltonian {th:.2f} s (medians of 3); scaled x{scale:.0f} by volume to one {L_full}^4 "
-                      f"trajectory ({4 * n_md} forces, {4 * n_md + 1} updates, 2 H) = {t_traj:.0f} s"}
+    t_warm, Fref = timed(lambda: O.get_force(dims, U, a))
+    fmax = np.abs(Fref).max()
+    frms = np.sqrt(np.mean(Fref ** 2))
+
+    def err(f):
+        return {"max_rel": float(np.abs(f - Fref).max() / fmax),
+                "rms_rel": float(np.sqrt(np.mean((f - Fref) ** 2)) / frms)}
+
+    e32, e64 = err(f32), err(f64)
+    del f32, f64
+    pref = O.avg_plaquette(dims, U)
+    errors = {"force_fp32": e32, "force_fp64": e64,
+              "plaq_oracle": pref, "plaq_rel_fp32_links": abs(p32 - pref) / abs(pref),
+              "plaq_rel_fp64_links": abs(p64 - pref) / abs(pref),
+              "links": f"{L}^4 after thermalisation, widened to FP64 and reunitarized (FP32 force: those "
+                       f"links rounded to FP32 12-real)"}
+    print(f"bench: force errors FP32 {e32['max_rel']:.2e} FP64 {e64['max_rel']:.2e}; timing the oracle",
+          file=sys.stderr, flush=True)
+    tf = statistics.median(timed(lambda: O.get_force(dims, U, a))[0] for _ in range(3))
+    del Fref
+    P = O.init_momentum(dims, 7, 0)
+    tu = statistics.median(timed(lambda: O.evolve_gauge_field(dims, U, P, 0.01, 0))[0] for _ in range(3))
+    th = statistics.median(timed(lambda: O.hamiltonian(dims, U, P, a))[0] for _ in range(3))
+    t_traj = 4 * n_md * tf + (4 * n_md + 1) * tu + 2 * th
+    base = {"value": round(1.0 / t_traj, 8), "unit": "traj/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"FP64 oracle on the thermalised {L}^4 links, same action: force {tf:.2f} s "
+                      f"(warm-up {t_warm:.2f} s), link update {tu:.3f} s, Hamiltonian {th:.2f} s (medians of 3); "
+                      f"one trajectory = {4 * n_md} forces + {4 * n_md + 1} updates + 2 H = {t_traj:.0f} s"}
+    return errors, base
 
 
-def serve_sample(M, ctx, nchains, max_traj):
+def serve_stream(M, ctx, nchains, max_traj):
     """The serving stream (Poisson arrivals at 200/s; volumes {4,6,8,12,16,24}^4; Wilson or Iwasaki;
     0-3 stout levels; own seeds; FP32 MD, n_md = 10) for its first `nchains` chains with trajectory
     counts drawn from 5..max_traj, through the continuous-batching server."""
     from mag_amd import serving as S
     stream = S.poisson_stream(nchains=nchains, rate=200.0, seed=2026, traj_range=(5, max_traj))
     srv = S.Server(ctx, hbm_budget=200 << 30, max_batch=64, graphs=True)
-    out, wall = S.drive(srv, stream)
+    out, wall = S.drive(srv, stream, log=lambda m: print("bench: serving " + m, file=sys.stderr, flush=True))
     st = S.latency_stats(out)
     srv.close()
+    total = sum(r.ntraj for _, r in stream)
     st.update({"chains": nchains, "wall_s": round(wall, 3), "traj_per_s": round(st["trajectories"] / wall, 2),
-               "sample": f"first {nchains} chains of the 512-chain stream, trajectories 5..{max_traj}"})
+               "sample": (f"the full {nchains}-chain stream, {total} trajectories" if nchains == 512 and max_traj == 50
+                          else f"first {nchains} chains of the 512-chain stream, trajectories 5..{max_traj}")})
     return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
 
 
@@ -146,25 +202,48 @@ def traffic_bytes(kernel, L):
         return None
 
 
-def thermalize(M, ctx, L, arg, seed, ntraj):
-    """Untimed warm-up: Haar start, `ntraj` FP32 trajectories of the same
-    workload with forced accepts. Returns the chain holding the links."""
-    ch = M.Chain(ctx, (L,) * 4, dataclasses.replace(arg, num_forced_accept_step=ntraj), seed=seed, precision=M.F32)
+# Thermalisation of the headline chain, independent of --warmup: at least
+# THERM_MIN forced-accept FP32 trajectories from the Haar start, then blocks
+# of THERM_BLOCK until the mean plaquette of the last block differs from the
+# block before by less than THERM_DRIFT (relative), at most THERM_MAX. The
+# measured chains accept within +-0.5% of their starting action; a drift of
+# 2e-4 per 10 trajectories keeps a 25-trajectory run well inside it.
+THERM_BLOCK, THERM_DRIFT, THERM_MAX = 10, 2e-4, 120
+
+
+def thermalize(M, ctx, L, arg, seed, tmin):
+    """Untimed: Haar start, then FP32 trajectories of the same workload with
+    forced accepts until the plaquette has stopped drifting. Returns the chain
+    holding the links and a summary."""
+    ch = M.Chain(ctx, (L,) * 4, dataclasses.replace(arg, num_forced_accept_step=THERM_MAX), seed=seed,
+                 precision=M.F32)
     ch.haar(seed)
-    ch.run(ntraj)
+    plaq = [r.avg_plaq for r in ch.run(max(tmin, 2 * THERM_BLOCK))]
+    while True:
+        last = sum(plaq[-THERM_BLOCK:]) / THERM_BLOCK
+        prev = sum(plaq[-2 * THERM_BLOCK:-THERM_BLOCK]) / THERM_BLOCK
+        drift = abs(last - prev) / abs(last)
+        if drift < THERM_DRIFT or len(plaq) + THERM_BLOCK > THERM_MAX:
+            break
+        plaq += [r.avg_plaq for r in ch.run(THERM_BLOCK)]
     ctx.synchronize()
-    return ch
+    print(f"bench: thermalised in {len(plaq)} trajectories, plaquette {plaq[-1]:.6f}, block drift {drift:.1e}",
+          file=sys.stderr, flush=True)
+    return ch, {"trajectories": len(plaq), "avg_plaq": round(plaq[-1], 8), "block_drift": drift}
 
 
-def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing, wloop=False):
+def timed_chain(M, ctx, L, arg, seed, precision, start, warmup, steps, timing, wloop=False):
     """A fresh chain (trajectory 0, Metropolis + demon window of 0.5% of the
-    initial plaquette sum) from the links of `start`: `steps` trajectories
-    timed together between two device synchronisations (with `wloop`, one
-    r = (4,0,0), T = 4 Wilson loop after each trajectory, inside the timed
-    region). Returns (seconds, results, chain, Wilson loops)."""
+    initial action sums) from the links of `start`: `warmup` untimed
+    trajectories, then `steps` trajectories timed together between two device
+    synchronisations (with `wloop`, one r = (4,0,0), T = 4 Wilson loop after
+    each trajectory, inside the timed region). Returns (seconds, warm-up
+    results, timed results, chain, Wilson loops)."""
     ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005)
     ch.links().copy_from(start.links())
+    ch.reset_demon()
     ch.demon_state()  # window centre from the starting links, outside the timed region
+    pre = ch.run(warmup) if warmup else []
     ctx.synchronize()
     if timing:
         ctx.reset_timing()
@@ -181,7 +260,37 @@ def timed_chain(M, ctx, L, arg, seed, precision, start, steps, timing, wloop=Fal
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
-    return dt, res, ch, loops
+    return dt, pre, res, ch, loops
+
+
+def paired_acceptance(r32, r64):
+    """The FP32-MD and FP64 chains start from the same links and draw the same
+    momenta and die rolls, so trajectory i of both integrates the same initial
+    state while every earlier decision agreed (the first trajectory whose
+    decisions differ still pairs). Over those trajectories: the Metropolis
+    probabilities min(1, e^-dH) of both, their paired difference (mean and
+    standard error), |dH32 - dH64| (absolute, relative to |H|) and the
+    plaquette difference of the resulting states."""
+    import math
+    m = min(len(r32), len(r64))
+    n = 0
+    while n < m and r32[n].accept == r64[n].accept:
+        n += 1
+    n = min(n + 1, m)
+    p32 = [min(1.0, math.exp(-r.dh)) for r in r32[:n]]
+    p64 = [min(1.0, math.exp(-r.dh)) for r in r64[:n]]
+    d = [x - y for x, y in zip(p32, p64)]
+    mean = sum(d) / n
+    se = (sum((x - mean) ** 2 for x in d) / (n - 1) / n) ** 0.5 if n > 1 else None
+    ddh = [abs(a.dh - b.dh) for a, b in zip(r32[:n], r64[:n])]
+    dpl = [abs(a.avg_plaq - b.avg_plaq) for a, b in zip(r32[:n], r64[:n]) if a.accept and b.accept]
+    return {"trajectories": n, "same_decisions": sum(a.accept == b.accept for a, b in zip(r32[:n], r64[:n])),
+            "metropolis_p_fp32_mean": round(sum(p32) / n, 5), "metropolis_p_fp64_mean": round(sum(p64) / n, 5),
+            "metropolis_p_diff_mean": mean, "metropolis_p_diff_se": se,
+            "dh_abs_diff_max": max(ddh), "dh_abs_diff_mean": sum(ddh) / n,
+            "dh_diff_rel_to_h_max": max(ddh) / abs(r64[0].h_old),
+            "plaq_abs_diff_max": max(dpl) if dpl else None,
+            "dh_fp32": [round(r.dh, 5) for r in r32[:n]], "dh_fp64": [round(r.dh, 5) for r in r64[:n]]}
 
 
 # Coupled coarse/fine md:: HMC: 3x3 products per coarse site of one fine
@@ -348,8 +457,8 @@ def main():
     nlinks = 4 * V
     print(f"bench: {L}^4 {act_name}+{a.n_stout}stout FP32-MD, warmup {a.warmup}, steps {a.steps}", file=sys.stderr,
           flush=True)
-    therm = thermalize(M, ctx, L, arg, a.seed, a.warmup)
-    dt, res, ch32, loops = timed_chain(M, ctx, L, arg, a.seed, M.F32, therm, a.steps, True, a.wloop)
+    therm, therm_info = thermalize(M, ctx, L, arg, a.seed, a.therm_min)
+    dt, pre, res, ch32, loops = timed_chain(M, ctx, L, arg, a.seed, M.F32, therm, a.warmup, a.steps, True, a.wloop)
     print(f"bench: FP32 {a.steps} trajectories in {dt:.3f} s", file=sys.stderr, flush=True)
     fams = {}
     for f in list(KERNEL_UNITS) + ["stout_fwd_h", "action", "kinetic", "reduce", "mom_refresh", "reunitarize"]:
@@ -387,7 +496,9 @@ def main():
     out = {"metric": "traj/s + force GB/s at 32^4 Iwasaki+3stout, FP32-MD vs FP64; chain p99 latency",
            "value": round(traj_s, 4), "unit": "traj/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32", "data": "synthetic (Haar-random links, seed %d)" % a.seed,
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (Haar-random links, seed %d, thermalised by %d forced-accept FP32 trajectories)"
+                   % (a.seed, therm_info["trajectories"]),
            "config": {"workload": f"{L}^4 {act_name}(c1={c1}, beta={beta}) + {a.n_stout}-stout(rho=0.1), "
                                   f"FG tau=1 n={n}, FP32 12-real links, FP64 dH, demon window 0.5%"
                                   + (", 4x4 Wilson loop per trajectory" if a.wloop else ""),
@@ -396,34 +507,32 @@ def main():
            "force_gbs": round(force_gbs, 1) if force_gbs else None,
            "force_tflops": round(force_tflops, 2) if force_tflops else None,
            "accept_rate": round(sum(r.accept for r in res) / len(res), 3),
+           "thermalisation": therm_info,
            "dh": [round(r.dh, 5) for r in res],
            "kernel_ms_per_traj": {f: round(v[0] / a.steps, 3) for f, v in fams.items()},
            "roofline": roof}
     if loops:
         out["wilson_loop_4x4"] = [round(w, 8) for w in loops]
+    if out["accept_rate"] == 0:
+        out["invalid"] = "the measured chain accepted no trajectory"
     if a.fp64_steps > 0:
-        dt64, r64, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.fp64_steps, False, a.wloop)
+        dt64, pre64, r64, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_steps, False,
+                                             a.wloop)
         out["fp64_traj_per_s"] = round(a.fp64_steps / dt64, 4)
         out["fp32_over_fp64"] = round(traj_s / (a.fp64_steps / dt64), 3)
         out["fp64_accept_rate"] = round(sum(r.accept for r in r64) / len(r64), 3)
-        m = min(len(r64), len(res))
-        same = 0
-        while same < m and res[same].accept == r64[same].accept:
-            same += 1
-        # While the accept decisions agree the two chains follow the same
-        # trajectories (same momenta), so their dH are comparable one by one.
-        mx = max((abs(res[i].dh - r64[i].dh) for i in range(same)), default=None)
-        out["dh_fp32_vs_fp64"] = {"trajectories": same, "max_abs": mx,
-                                  "max_rel_to_h": mx / abs(r64[0].h_old) if mx is not None else None,
-                                  "fp64_dh": [round(r.dh, 5) for r in r64]}
+        # warm-up and timed trajectories of both chains, paired one by one
+        out["fp32_vs_fp64"] = paired_acceptance(pre + res, pre64 + r64)
         print(f"bench: FP64 {a.fp64_steps} trajectories in {dt64:.3f} s", file=sys.stderr, flush=True)
     if a.reversibility:
         dev, dh_rt = ch32.reversibility()
         out["reversibility_fp64"] = {"max_link_dev": dev, "dh_roundtrip": dh_rt}
+    del ch32
     if a.serve_chains > 0:
-        out["serving"] = serve_sample(M, ctx, a.serve_chains, a.serve_max_traj)
+        out["serving"] = serve_stream(M, ctx, a.serve_chains, a.serve_max_traj)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(L, n, a.n_stout)
+        errors, out["cpu_baseline"] = oracle_leg(M, ctx, L, arg, kind, beta, c1, a.n_stout, n, therm)
+        out["errors_vs_oracle"] = errors
     print(json.dumps(out))
 
 

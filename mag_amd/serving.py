@@ -125,14 +125,20 @@ def poisson_stream(nchains=512, rate=200.0, seed=2026, volumes=(4, 6, 8, 12, 16,
     return out
 
 
-def drive(server, stream, time_scale=1.0):
+def drive(server, stream, time_scale=1.0, log=None, log_every=20.0):
     """Submit each request once its arrival time (scaled) has passed, stepping
-    the server meanwhile. Returns {id: (arrival_s, t_done list, results)}."""
+    the server meanwhile; `log` (if given) receives a progress line every
+    `log_every` seconds. Returns {id: (arrival_s, t_done list, results)}."""
     t0 = time.perf_counter()
     info0 = server.info()[2]
     ids, k = {}, 0
+    next_log = log_every
     while True:
         now = time.perf_counter() - t0
+        if log is not None and now >= next_log:
+            u, ntr, _ = server.info()
+            log(f"{now:.0f} s: {k}/{len(stream)} submitted, {ntr} trajectories done, {u / 2**30:.1f} GiB in use")
+            next_log += log_every
         while k < len(stream) and stream[k][0] * time_scale <= now:
             ids[server.submit(stream[k][1])] = stream[k][0] * time_scale
             k += 1
