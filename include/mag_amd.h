@@ -168,6 +168,20 @@ int mag_total_rectangular_async(mag_field U, double* d_out);
 int mag_hamiltonian_async(mag_field U, mag_field P, const mag_action* act, double* d_out);
 int mag_avg_wilson_loop_async(mag_field U, const int r[3], int T, double* d_out);
 
+/* Allocation-free Hamiltonian. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It computes what
+ * mag_hamiltonian_async computes for a thin action (n_stout = 0), with the
+ * same kernels in the same order (d_out[0..2] = {kinetic, potential, H}),
+ * but its partial sums live in a fixed device array of the library (1.5 MiB
+ * per GPU, held until the process ends) instead of context workspace, and it
+ * allocates nothing. Lattices above MAG_NOALLOC_MAX_SITES sites return
+ * MAG_ERR_ARG; smeared actions MAG_ERR_UNSUPPORTED. Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (other streams,
+ * contexts or threads) race, and the caller must serialise them. */
+#define MAG_NOALLOC_MAX_SITES 16777216L /* 64^4 */
+int mag_hamiltonian_noalloc(mag_field U, mag_field P, const mag_action* act, double* d_out);
+
 /* -------------------------------------------------------------- HMC chain
  * run_chmc / update_field (field-hmc.h:770-986) as a device-resident chain:
  * links, backup, momenta, workspace and RNG state stay in HBM. */

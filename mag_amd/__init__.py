@@ -23,7 +23,7 @@ __all__ = [
     "reunitarize", "init_momentum", "expiQ", "compute_Lambda", "stout_smear", "avg_wilson_loop",
     "demon_microcanonical", "run_chmc", "Arg_export", "export_config_nersc", "import_config_nersc",
     "nersc_write", "nersc_read", "total_plaq_async", "total_rectangular_async", "get_hamiltonian_async",
-    "avg_wilson_loop_async", "coarsen_links", "update_constrain", "check_constrained_plaquette",
+    "avg_wilson_loop_async", "get_hamiltonian_noalloc", "coarsen_links", "update_constrain", "check_constrained_plaquette",
     "lattice_expansion", "GchbArg", "AlgGCtrnHeatBath", "local_metropolis", "ORDER_REFERENCE", "ORDER_CHECKERBOARD",
     "demon_checkerboard",
 ]
@@ -292,6 +292,14 @@ def get_hamiltonian_async(U, P, arg, d_out):
     """d_out[0:3] = (kinetic, potential, H)."""
     a = _act(arg)
     _check(_lib.lib().mag_hamiltonian_async(U.handle, P.handle, C.byref(a), C.c_void_p(d_out)), U.ctx.handle)
+
+
+def get_hamiltonian_noalloc(U, P, arg, d_out):
+    """get_hamiltonian_async of a thin action without any allocation (partial
+    sums in the library's fixed per-GPU array): d_out[0:3] = (kinetic,
+    potential, H). Calls overlapping in time on one GPU race."""
+    a = _act(arg)
+    _check(_lib.lib().mag_hamiltonian_noalloc(U.handle, P.handle, C.byref(a), C.c_void_p(d_out)), U.ctx.handle)
 
 
 def avg_wilson_loop_async(U, r, T, d_out):

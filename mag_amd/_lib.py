@@ -104,6 +104,7 @@ SIGNATURES = {
     "mag_total_rectangular_async": [_P, _P],
     "mag_hamiltonian_async": [_P, _P, C.POINTER(Action), _P],
     "mag_avg_wilson_loop_async": [_P, _I, C.c_int, _P],
+    "mag_hamiltonian_noalloc": [_P, _P, C.POINTER(Action), _P],
     "mag_chain_create": [_P, _I, C.POINTER(Action), C.POINTER(HmcParams), C.POINTER(_P)],
     "mag_chain_destroy": [_P],
     "mag_chain_set_links_aos": [_P, _D],

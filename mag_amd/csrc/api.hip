@@ -7,6 +7,7 @@
 #include <cstring>
 #include <vector>
 
+#include "kernels.hpp"
 #include "runtime.hpp"
 #include "su3.hpp"
 
@@ -125,6 +126,11 @@ int mag_ctx_create(int device, mag_ctx* out) {
   if (hipSetDevice(device) != hipSuccess) return MAG_ERR_DEVICE;
   auto* c = new mag_ctx_s();
   c->device = device;
+  // device address of the allocation-free entry point's scratch (once per device)
+  if (noalloc_setup(device, &c->err) != MAG_OK) {
+    delete c;
+    return MAG_ERR_DEVICE;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return MAG_ERR_DEVICE;
@@ -471,23 +477,12 @@ int mag_avg_wilson_loop(mag_field U, const int r3[3], int T, double* out) {
 namespace {
 // out[i] = sum_j w[i][j] s[j] over the context's reduction slots, written to
 // caller memory in stream order.
-struct LinCombArgs {
-  const double* s;
-  double* out;
-  int nout;
-  double w[3][4];
-};
-__global__ void k_lincomb(LinCombArgs a) {
-  const int i = threadIdx.x;
-  if (i < a.nout) a.out[i] = a.w[i][0] * a.s[0] + a.w[i][1] * a.s[1] + a.w[i][2] * a.s[2] + a.w[i][3] * a.s[3];
-}
-
 int lincomb(mag_ctx c, double* d_out, int nout, const double w[3][4]) {
   if (!d_out) return MAG_ERR_ARG;
   LinCombArgs a{c->d_scalar, d_out, nout, {}};
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 4; ++j) a.w[i][j] = w[i][j];
-  hipLaunchKernelGGL(k_lincomb, dim3(1), dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL(k_lincomb<0>, dim3(1), dim3(64), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return MAG_OK;
 }

@@ -910,6 +910,21 @@ __global__ __launch_bounds__(BLOCK) void k_final_reduce(ReduceArgs a) { k_final_
 template <int OP>
 __global__ __launch_bounds__(BLOCK) void k_final_reduce_b(const ReduceArgs* t) { k_final_reduce_body<OP>(t[blockIdx.z]); }
 
+// out[i] = sum_j w[i][j] s[j] for i < nout (<= 3): linear combinations of
+// reduction results (e.g. kinetic, potential and H from the plaquette,
+// rectangle and momentum sums), written to caller memory. One block of 64.
+struct LinCombArgs {
+  const double* s;
+  double* out;
+  int nout;
+  double w[3][4];
+};
+template <int DUMMY>
+__global__ void k_lincomb(LinCombArgs a) {
+  const int i = threadIdx.x;
+  if (i < a.nout) a.out[i] = a.w[i][0] * a.s[0] + a.w[i][1] * a.s[1] + a.w[i][2] * a.s[2] + a.w[i][3] * a.s[3];
+}
+
 // Device copy and fill (used instead of runtime memcpy / memset nodes so a
 // trajectory can be batched over chains and captured in a graph).
 struct CopyArgs {

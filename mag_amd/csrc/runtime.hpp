@@ -368,6 +368,8 @@ int convert_links(mag_ctx c, const Geom& g, int prec_in, int kind_in, void* in, 
 int batch_expiQ(mag_ctx c, const double* dQ, double* dout, long n, int prec);
 int batch_lambda(mag_ctx c, const double* dQ, const double* dS, const double* dU, double* dout, long n, int prec);
 double host_uniform(uint64_t seed, uint32_t stream, uint64_t index);
+// noalloc.hip: resolve the allocation-free scratch array of `device` (current)
+int noalloc_setup(int device, std::string* err);
 
 inline double action_from_sums(const mag_action& act, double sp, double sr) {
   // field-hmc.h:632-640
