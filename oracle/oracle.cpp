@@ -17,6 +17,15 @@
 //  * Reductions sum per-site values serially in site order (deterministic
 //    for any thread count) instead of per-thread partial sums.
 //  * A NaN Hamiltonian is rejected even during forced-accept trajectories.
+//
+// Every function is pinned by a check outside this file (tests/test_oracle.py,
+// tests/test_oracle_pins.py, tests/test_multi_oracle.py,
+// tests/test_local_oracle.py): scipy expm, Random123 known answers, finite
+// differences, independent numpy restatements (lattice sums, the stout forward
+// map and get_Q of hep-lat/0311018 eqs. 1-3, brute-force Wilson loops), the
+// chair/twist tables' endpoint structure, invariants and closed forms. Parity
+// unpinned: the qlat random streams this file replaces with Philox (no qlat
+// here), and CPS operation order at the rounding level.
 #include "oracle.h"
 #include "su3.hpp"
 
@@ -1376,6 +1385,16 @@ ORC_SITE_FN(orc_staple_dagger, staple_dagger(f, x, mu))
 ORC_SITE_FN(orc_rectangular_dagger, rectangular_dagger(f, x, mu))
 ORC_SITE_FN(orc_chair_staple_dagger, chair_staple(f, x, mu))
 ORC_SITE_FN(orc_twist_staple_dagger, twist_staple(f, x, mu))
+
+// The chair (twist = 0) or twist (twist = 1) path table of direction mu:
+// writes up to max paths of 5 direction codes each and returns the count.
+int orc_staple_index(int mu, int twist, int* out, int max) {
+  const auto& t = twist ? twist_index(mu) : chair_index(mu);
+  const int n = (int)t.size();
+  for (int i = 0; i < n && i < max; ++i)
+    for (int k = 0; k < 5; ++k) out[5 * i + k] = t[i][k];
+  return n;
+}
 
 int orc_is_constrained(const int* x, int mu, int mag) { return is_constrained(x, mu, mag); }
 

@@ -72,6 +72,8 @@ void orc_path_product(const int* L, const double* U, const int* x, const int* di
 void orc_staple_dagger(const int* L, const double* U, long site, int mu, double* out);
 void orc_rectangular_dagger(const int* L, const double* U, long site, int mu, double* out);
 void orc_chair_staple_dagger(const int* L, const double* U, long site, int mu, double* out);
+// chair (twist = 0) / twist (1) path table of mu: up to max paths of 5 codes; returns the count
+int orc_staple_index(int mu, int twist, int* out, int max);
 void orc_twist_staple_dagger(const int* L, const double* U, long site, int mu, double* out);
 int orc_is_constrained(const int* x, int mu, int mag);
 

@@ -80,6 +80,7 @@ def _declare(l):
         "orc_staple_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
         "orc_rectangular_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
         "orc_chair_staple_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
+        "orc_staple_index": (C.c_int, [C.c_int, C.c_int, _I, C.c_int]),
         "orc_twist_staple_dagger": (None, [_I, _D, C.c_long, C.c_int, _D]),
         "orc_is_constrained": (C.c_int, [_I, C.c_int, C.c_int]),
         "orc_total_plaq": (C.c_double, [_I, _D]),
@@ -249,6 +250,14 @@ def chair_staple_dagger(L, U, site, mu):
 
 def twist_staple_dagger(L, U, site, mu):
     return _site(lib().orc_twist_staple_dagger, L, U, site, mu)
+
+
+def staple_index(mu, twist):
+    """The oracle's chair (twist=False) or twist path table of direction mu:
+    a list of 5-step direction-code paths (codes as path_product takes them)."""
+    buf = (C.c_int * (5 * 256))()
+    n = lib().orc_staple_index(mu, 1 if twist else 0, buf, 256)
+    return [list(buf[5 * i:5 * i + 5]) for i in range(n)]
 
 
 def is_constrained(x, mu, mag):
