@@ -128,6 +128,9 @@ int mag_expiQ_batch(mag_ctx ctx, const double* Q, double* out, long n, int prec)
 /* compute_Lambda (field-multi.h:180-269), batch of n; Q Hermitian traceless */
 int mag_compute_Lambda_batch(mag_ctx ctx, const double* Q, const double* SigmaP, const double* U, double* out,
                              long n, int prec);
+/* DexpiQ (field-multi.h:90-178), batch of n: d exp(iQ(t))/dt for dQ/dt = DQ,
+ * zero when Q = 0 exactly (as the reference); Q Hermitian traceless */
+int mag_DexpiQ_batch(mag_ctx ctx, const double* Q, const double* DQ, double* out, long n, int prec);
 
 /* ------------------------------------------------------- forces and updates */
 /* get_force (field-hmc.h:339-425); with n_stout > 0 the force of the smeared
@@ -215,6 +218,13 @@ int mag_chain_traj_index(mag_chain c, long* next_traj);
 int mag_demon_microcanonical(mag_field U, int nsweeps, double E_max, uint64_t seed, double E[4],
                              long acc_rej[2]);
 
+/* chair_staple_dagger / twist_staple_dagger (field-staple.h:141-165) for every
+ * link: S(x, mu) = the sum over the 72 chair (twist) paths of field-staple.h:
+ * 27-135, defects kept, returned undaggered as the reference returns it. U
+ * and S are FP64 MAG_LINK18 fields of one shape (S holds general matrices). */
+int mag_chair_staple_dagger(mag_field S, mag_field U);
+int mag_twist_staple_dagger(mag_field S, mag_field U);
+
 /* Checkerboard demon: demon_microcanonical as a parallel update with a demon
  * per link. Ep, Er, Ec, Et are MAG_REAL1 fields of U's shape holding every
  * link's plaquette, rectangle, chair and twist demon energies (in/out; start
@@ -278,6 +288,9 @@ typedef struct {
 
 /* get_xi_energy (field-multi-md.h:30-72) */
 int mag_xi_energy(mag_field UF, mag_field UC, mag_field xi, const mag_multi_params* mp, double* out);
+/* get_Q (field-multi.h:32-39) for every coarse link: Q = -i rho TA(stp U2^+)
+ * into a coarse FP64 MAG_ALG8 field (Hermitian traceless, 8 components) */
+int mag_multi_get_Q(mag_field Q, mag_field UF, const mag_multi_params* mp);
 /* get_Gb (field-multi-md.h:771-783): Gb = e^{iQ} U2 into a coarse link field */
 int mag_multi_gb(mag_field Gb, mag_field UF, const mag_multi_params* mp);
 /* md::get_Fforce (field-multi-md.h:116-368): thin gauge force (no link

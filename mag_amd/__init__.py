@@ -20,7 +20,8 @@ __all__ = [
     "Context", "Field", "Gauge", "Arg_chmc", "Chain", "MagError",
     "get_force", "evolve_gauge_field", "evolve_momentum", "force_gradient_integrator", "leap_frog_integrator",
     "get_hamiltonian", "total_plaq", "total_rectangular", "avg_plaquette", "gauge_action", "kinetic_energy",
-    "reunitarize", "init_momentum", "expiQ", "compute_Lambda", "stout_smear", "avg_wilson_loop",
+    "reunitarize", "init_momentum", "expiQ", "compute_Lambda", "DexpiQ", "chair_staple_dagger",
+    "twist_staple_dagger", "stout_smear", "avg_wilson_loop",
     "demon_microcanonical", "run_chmc", "Arg_export", "export_config_nersc", "import_config_nersc",
     "nersc_write", "nersc_read", "total_plaq_async", "total_rectangular_async", "get_hamiltonian_async",
     "avg_wilson_loop_async", "get_hamiltonian_noalloc", "coarsen_links", "update_constrain", "check_constrained_plaquette",
@@ -324,6 +325,24 @@ def expiQ(ctx, Q, prec=F64):
 def compute_Lambda(ctx, Q, SigmaP, U, prec=F64):
     """compute_Lambda (field-multi.h:180-269) on the device for a batch."""
     return _batch(ctx, _lib.lib().mag_compute_Lambda_batch, [Q, SigmaP, U], prec)
+
+
+def DexpiQ(ctx, Q, DQ, prec=F64):
+    """DexpiQ (field-multi.h:90-178) on the device for a batch: d exp(iQ)/dt along DQ."""
+    return _batch(ctx, _lib.lib().mag_DexpiQ_batch, [Q, DQ], prec)
+
+
+def chair_staple_dagger(S, U):
+    """chair_staple_dagger (field-staple.h:141-151) for every link into the FP64
+    18-real field S (the undaggered chair sum, as the reference returns it)."""
+    _check(_lib.lib().mag_chair_staple_dagger(S.handle, U.handle), U.ctx.handle)
+    return S
+
+
+def twist_staple_dagger(S, U):
+    """twist_staple_dagger (field-staple.h:156-165) for every link into S (undaggered)."""
+    _check(_lib.lib().mag_twist_staple_dagger(S.handle, U.handle), U.ctx.handle)
+    return S
 
 
 def demon_microcanonical(U, nsweeps, E_max, seed, E=None):

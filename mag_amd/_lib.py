@@ -105,6 +105,10 @@ SIGNATURES = {
     "mag_hamiltonian_async": [_P, _P, C.POINTER(Action), _P],
     "mag_avg_wilson_loop_async": [_P, _I, C.c_int, _P],
     "mag_hamiltonian_noalloc": [_P, _P, C.POINTER(Action), _P],
+    "mag_DexpiQ_batch": [_P, _D, _D, _D, C.c_long, C.c_int],
+    "mag_chair_staple_dagger": [_P, _P],
+    "mag_twist_staple_dagger": [_P, _P],
+    "mag_multi_get_Q": [_P, _P, C.POINTER(MultiParams)],
     "mag_chain_create": [_P, _I, C.POINTER(Action), C.POINTER(HmcParams), C.POINTER(_P)],
     "mag_chain_destroy": [_P],
     "mag_chain_set_links_aos": [_P, _D],

@@ -101,8 +101,9 @@ int batch_hook(mag_ctx c, long n, int prec, const std::vector<const double*>& in
         r = fail(c, MAG_ERR_DEVICE, "hook upload failed");
     if (r == MAG_OK) {
       double* o = (double*)d.back();
-      r = which == 0 ? batch_expiQ(c, (const double*)d[0], o, n, prec)
-                     : batch_lambda(c, (const double*)d[0], (const double*)d[1], (const double*)d[2], o, n, prec);
+      r = which == 0   ? batch_expiQ(c, (const double*)d[0], o, n, prec)
+          : which == 1 ? batch_lambda(c, (const double*)d[0], (const double*)d[1], (const double*)d[2], o, n, prec)
+                       : batch_dexpiq(c, (const double*)d[0], (const double*)d[1], o, n, prec);
       if (r == MAG_OK && (hipMemcpyAsync(out, o, b, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                           hipStreamSynchronize(c->stream) != hipSuccess))
         r = fail(c, MAG_ERR_DEVICE, "hook download failed");
@@ -323,6 +324,11 @@ int mag_compute_Lambda_batch(mag_ctx c, const double* Q, const double* S, const 
                              int prec) {
   if (!Q || !S || !U) return MAG_ERR_ARG;
   return batch_hook(c, n, prec, {Q, S, U}, out, 1);
+}
+
+int mag_DexpiQ_batch(mag_ctx c, const double* Q, const double* DQ, double* out, long n, int prec) {
+  if (!Q || !DQ) return MAG_ERR_ARG;
+  return batch_hook(c, n, prec, {Q, DQ}, out, 2);
 }
 
 int mag_gauge_force(mag_field F, mag_field U, const mag_action* act) {

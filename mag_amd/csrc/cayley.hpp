@@ -273,6 +273,31 @@ MAG_HD M3<T> exp_right(const M3<T>& U, const M3<T>& Q) {
   }
 }
 
+// DexpiQ (field-multi.h:90-178): dE = d exp(iQ(t))/dt for dQ/dt = DQ, from
+// B1, B2 (MP eqs. 69-70): dE = Tr(Q DQ) B1 + Tr(Q^2 DQ) B2 + f1 DQ
+// + f2 (DQ Q + Q DQ), the B terms folded into coefficients of 1, Q, Q^2.
+// Q is Hermitian traceless. Zero when c1 == 0 (exactly Q = 0), as in the
+// reference. E = exp(iQ) comes with it.
+template <class T>
+MAG_HD void dexpiq(const M3<T>& Q, const M3<T>& DQ, M3<T>& E, M3<T>& dE) {
+  const M3<T> Q2 = mul(Q, Q);
+  T c0, c1;
+  ch_invariants(Q, Q2, c0, c1);
+  const CHC<T> c = ch_coef<true>(c0, c1);
+  if (c.zero) {
+    E = unit3<T>();
+    dE = zero3<T>();
+    return;
+  }
+  E = ch_combine(Q, Q2, c.f0, c.f1, c.f2);
+  const cpx<T> t1 = trace_mul(Q, DQ), t2 = trace_mul(Q2, DQ);
+  dE = ch_combine(Q, Q2, t1 * c.b10 + t2 * c.b20, t1 * c.b11 + t2 * c.b21, t1 * c.b12 + t2 * c.b22);
+  M3<T> QD = mul_hn(Q, DQ);
+  mul_acc_nh(QD, DQ, Q);  // DQ Q + Q DQ
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dE.e[k] = cfma(c.f1, DQ.e[k], cfma(c.f2, QD.e[k], dE.e[k]));
+}
+
 // Lambda of compute_Lambda (field-multi.h:180-269), returned as its 8
 // algebra components, given Q and X = U SigmaP (so Tr(SigmaP B U) = Tr(X B)).
 // Also returns exp(iQ) for the chain's Z' e^{iQ} term.

@@ -92,6 +92,16 @@ int batch_expiQ(mag_ctx c, const double* dQ, double* dout, long n, int prec) {
   return MAG_OK;
 }
 
+int batch_dexpiq(mag_ctx c, const double* dQ, const double* dDQ, double* dout, long n, int prec) {
+  const dim3 grid((unsigned)((n + BLOCK - 1) / BLOCK));
+  if (prec == MAG_F64)
+    hipLaunchKernelGGL(k_dexpiq_batch<double>, grid, dim3(BLOCK), 0, c->stream, dQ, dDQ, dout, n);
+  else
+    hipLaunchKernelGGL(k_dexpiq_batch<float>, grid, dim3(BLOCK), 0, c->stream, dQ, dDQ, dout, n);
+  CHECK_LAUNCH(c);
+  return MAG_OK;
+}
+
 int batch_lambda(mag_ctx c, const double* dQ, const double* dS, const double* dU, double* dout, long n, int prec) {
   const dim3 grid((unsigned)((n + BLOCK - 1) / BLOCK));
   if (prec == MAG_F64)

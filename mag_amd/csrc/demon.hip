@@ -203,6 +203,41 @@ __global__ __launch_bounds__(64) void k_demon_sweep(LinkView<double, 18> U, Geom
   }
 }
 
+// ------------------------------------------------------------ chair / twist staples
+// chair_staple_dagger / twist_staple_dagger (field-staple.h:141-165) for every
+// link: S(x, mu) = the sum of the 72 chair (kind 2) or twist (kind 3) path
+// products of the table, undaggered as the reference returns it.
+__global__ __launch_bounds__(BLOCK) void k_staple_table_sum(LinkView<double, 18> U, Geom g, const DemonTable* tab,
+                                                            int kind, LinkView<double, 18> S) {
+  const int s = blockIdx.x * BLOCK + threadIdx.x;
+  const int mu = blockIdx.y;
+  if (s >= g.V) return;
+  const Site p0 = site_of(g, s);
+  M3<double> acc = zero3<double>();
+  for (int q = 0; q < NPATH; ++q) {
+    if (tab->kind[q] != kind) continue;
+    Site y = p0;
+    M3<double> m = unit3<double>();
+    for (int j = 0; j < PLEN; ++j) {
+      const int d = tab->dirs[mu][q][j];
+      if (d < 0) break;
+      if (d < 4) {
+        m = mul(m, U.template load<double>(d, y.s));
+        y.s += off_rt(g, y, d, 1);
+      } else {
+        y.s += off_rt(g, y, d - 4, -1);
+      }
+      y.x[0] = y.s % g.L[0];
+      y.x[1] = (y.s / g.S[1]) % g.L[1];
+      y.x[2] = (y.s / g.S[2]) % g.L[2];
+      y.x[3] = y.s / g.S[3];
+      if (d >= 4) m = mul_nd(m, U.template load<double>(d - 4, y.s));
+    }
+    add_to(acc, m);
+  }
+  S.store(mu, s, acc);
+}
+
 // ------------------------------------------------------------ checkerboard demon
 // demon_microcanonical with a demon per link (four energies per link, four
 // MAG_REAL1 fields), updated in parallel: per direction MU, 128 classes
@@ -464,3 +499,36 @@ extern "C" int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, m
   }
   return MAG_OK;
 }
+
+namespace {
+int staple_table_field(mag_field S, mag_field U, int kind) {
+  if (!S || !U || S->ctx != U->ctx) return MAG_ERR_ARG;
+  mag_ctx c = U->ctx;
+  if (U->kind != MAG_LINK18 || U->prec != MAG_F64 || S->kind != MAG_LINK18 || S->prec != MAG_F64)
+    return fail(c, MAG_ERR_UNSUPPORTED, "chair/twist staples take and give FP64 18-real fields");
+  for (int i = 0; i < 4; ++i)
+    if (S->dims[i] != U->dims[i]) return fail(c, MAG_ERR_ARG, "chair/twist staples: fields differ in shape");
+  if (S->p == U->p) return fail(c, MAG_ERR_ARG, "chair/twist staples: output must not alias the links");
+  if (c->rec) return fail(c, MAG_ERR_UNSUPPORTED, "chair/twist staples are not recordable");
+  DemonTable* ht = new DemonTable();
+  build_table(*ht);
+  DemonTable* dt = nullptr;
+  int r = dev_alloc(c, (void**)&dt, sizeof(DemonTable));
+  if (!r && hipMemcpyAsync(dt, ht, sizeof(DemonTable), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    r = fail(c, MAG_ERR_DEVICE, "staple table upload failed");
+  if (!r) {
+    KTimer t(c, "staple_table");
+    hipLaunchKernelGGL(k_staple_table_sum, grid_links(U->g), dim3(BLOCK), 0, c->stream,
+                       LinkView<double, 18>{(double*)U->p, U->g.V}, U->g, (const DemonTable*)dt, kind,
+                       LinkView<double, 18>{(double*)S->p, U->g.V});
+    if (hipGetLastError() != hipSuccess) r = fail(c, MAG_ERR_DEVICE, "staple table launch failed");
+  }
+  (void)hipStreamSynchronize(c->stream);
+  dev_free(c, dt, dt ? sizeof(DemonTable) : 0);
+  delete ht;
+  return r;
+}
+}  // namespace
+
+extern "C" int mag_chair_staple_dagger(mag_field S, mag_field U) { return staple_table_field(S, U, 2); }
+extern "C" int mag_twist_staple_dagger(mag_field S, mag_field U) { return staple_table_field(S, U, 3); }

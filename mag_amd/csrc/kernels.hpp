@@ -1017,6 +1017,25 @@ __global__ __launch_bounds__(BLOCK) void k_expiQ_batch(const double* Q, double* 
 }
 
 template <class TC>
+__global__ __launch_bounds__(BLOCK) void k_dexpiq_batch(const double* Q, const double* DQ, double* out, long n) {
+  const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  M3<TC> q, dq;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    q.e[k] = {TC(Q[18 * i + 2 * k]), TC(Q[18 * i + 2 * k + 1])};
+    dq.e[k] = {TC(DQ[18 * i + 2 * k]), TC(DQ[18 * i + 2 * k + 1])};
+  }
+  M3<TC> e, de;
+  dexpiq(q, dq, e, de);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    out[18 * i + 2 * k] = double(de.e[k].r);
+    out[18 * i + 2 * k + 1] = double(de.e[k].i);
+  }
+}
+
+template <class TC>
 __global__ __launch_bounds__(BLOCK) void k_lambda_batch(const double* Q, const double* Sg, const double* U,
                                                         double* out, long n) {
   const long i = (long)blockIdx.x * BLOCK + threadIdx.x;

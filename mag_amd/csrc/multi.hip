@@ -510,25 +510,8 @@ MAG_D void cforce_link(const MultiArgs& a, const int* x0, int sc) {
   w.fwd(MU);
   const M3<D> U2 = w.m, DU2 = w.d;
   // DexpiQ (field-multi.h:90-178); zero when c1 == 0, as in the reference
-  const M3<D> Q2 = mul(Q, Q);
-  D c0, c1;
-  ch_invariants(Q, Q2, c0, c1);
-  const CHC<D> c = ch_coef<true>(c0, c1);
-  // dE = Tr(Q DQ) B1 + Tr(Q^2 DQ) B2 + f1 DQ + f2 (DQ Q + Q DQ), with the
-  // B1, B2 terms folded into three coefficients of 1, Q, Q^2
   M3<D> E, dE;
-  if (c.zero) {
-    E = unit3<D>();
-    dE = zero3<D>();
-  } else {
-    E = ch_combine(Q, Q2, c.f0, c.f1, c.f2);
-    const cpx<D> t1 = trace_mul(Q, DQ), t2 = trace_mul(Q2, DQ);
-    dE = ch_combine(Q, Q2, t1 * c.b10 + t2 * c.b20, t1 * c.b11 + t2 * c.b21, t1 * c.b12 + t2 * c.b22);
-    M3<D> QD = mul_hn(Q, DQ);
-    mul_acc_nh(QD, DQ, Q);  // DQ Q + Q DQ
-#pragma unroll
-    for (int k = 0; k < 9; ++k) dE.e[k] = cfma(c.f1, DQ.e[k], cfma(c.f2, QD.e[k], dE.e[k]));
-  }
+  dexpiq(Q, DQ, E, dE);
   // (DexpiQ U2 + e^{iQ} DU2) (e^{iQ} U2)^+, times -i alpha/(1+alpha)
   M3<D> V = mul(dE, U2);
   mul_acc(V, E, DU2);
@@ -567,6 +550,21 @@ __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_gb(MultiArgs a) {
   with_mu(blockIdx.y, [&](auto mc) {
     constexpr int MU = decltype(mc)::value;
     a.out.store(MU, sc, coarse_Gb<MU>(a, x0));
+  });
+}
+
+// get_Q (field-multi.h:33-39) per coarse link, stored as its 8 generator components
+__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_getq(MultiArgs a) {
+  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  if (sc >= a.gc.V) return;
+  int x0[4];
+  coarse_site(a, sc, x0);
+  with_mu(blockIdx.y, [&](auto mc) {
+    constexpr int MU = decltype(mc)::value;
+    D q[8];
+    M3<D> U2;
+    coarse_Q<MU>(a, x0, q, U2);
+    a.F.store(MU, sc, q);
   });
 }
 
@@ -870,6 +868,15 @@ int mag_multi_gb(mag_field Gb, mag_field UF, const mag_multi_params* mp) {
   MultiArgs a = args(m, UF->p, nullptr, nullptr);
   a.out = LV{(double*)Gb->p, m.gc.V};
   return mlaunch(m, "multi_gb", k_multi_gb, cgrid(m, 4), a);
+}
+
+int mag_multi_get_Q(mag_field Q, mag_field UF, const mag_multi_params* mp) {
+  Multi m;
+  int r;
+  if ((r = make_multi(UF, mp, 0, &m)) || (r = check_coarse(m, Q, f64alg))) return r;
+  MultiArgs a = args(m, UF->p, nullptr, nullptr);
+  a.F = AV{(double*)Q->p, m.gc.V};
+  return mlaunch(m, "multi_getq", k_multi_getq, cgrid(m, 4), a);
 }
 
 int mag_multi_fforce(mag_field F, mag_field UF, mag_field UC, mag_field xi, const mag_action* act,

@@ -367,6 +367,7 @@ int alg_scale(mag_ctx c, const Geom& g, int prec, void* P, double s);
 int convert_links(mag_ctx c, const Geom& g, int prec_in, int kind_in, void* in, int prec_out, int kind_out, void* out);
 int batch_expiQ(mag_ctx c, const double* dQ, double* dout, long n, int prec);
 int batch_lambda(mag_ctx c, const double* dQ, const double* dS, const double* dU, double* dout, long n, int prec);
+int batch_dexpiq(mag_ctx c, const double* dQ, const double* dDQ, double* dout, long n, int prec);
 double host_uniform(uint64_t seed, uint32_t stream, uint64_t index);
 // noalloc.hip: resolve the allocation-free scratch array of `device` (current)
 int noalloc_setup(int device, std::string* err);

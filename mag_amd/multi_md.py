@@ -48,6 +48,12 @@ def get_xi_energy(FgField, CgField, CxField, mp):
     return v.value
 
 
+def get_Q(CQField, FgField, mp):
+    """get_Q (field-multi.h:32-39) for every coarse link into a coarse FP64 algebra field."""
+    _check(_lib.lib().mag_multi_get_Q(CQField.handle, FgField.handle, C.byref(mp)), FgField.ctx.handle)
+    return CQField
+
+
 def get_Gb(CGbField, FgField, mp):
     """get_Gb (field-multi-md.h:771-783)."""
     _check(_lib.lib().mag_multi_gb(CGbField.handle, FgField.handle, C.byref(mp)), FgField.ctx.handle)

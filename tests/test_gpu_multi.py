@@ -77,6 +77,14 @@ def test_gb_and_xi_energy(ctx, orc):
     assert abs(s - ref) < 1e-12 * abs(ref)
 
 
+def test_get_Q(ctx, orc):
+    """get_Q (field-multi.h:32-39) at every coarse link against the oracle."""
+    U, Uc, xi = _random_state(ctx, orc, 13)
+    f = link(ctx, LF, U)
+    Q = MD.get_Q(M.Field(ctx, _coarse(LF), ALG8, F64), f, MD.params(0.15, 1.0)).download()
+    assert np.abs(Q - orc.multi_Q(LF, U, orc.multi(0.15, 1.0, 0.5))).max() < 1e-13
+
+
 @pytest.mark.parametrize("gauge", [M.Gauge(M.WILSON, 0.0), M.Gauge(M.IWASAKI, -0.331)])
 def test_fine_force(ctx, orc, gauge):
     U, Uc, xi = _random_state(ctx, orc, 21)

@@ -52,6 +52,35 @@ def test_alone_equals_mix_and_graph_equals_eager(ctx):
         srv.close()
 
 
+def test_c4_shaped_mix_alone_equals_mix(ctx):
+    """Chains of the serving workload's larger shapes (12^4 to 24^4, Wilson and
+    Iwasaki, 0-3 stout levels, FP32 and FP64, staggered arrivals): every
+    chain's results are bitwise the same alone and in the mix."""
+    reqs = [
+        S.request((24, 24, 24, 24), M.IWASAKI, 2.55, -0.331, n_stout=3, ntraj=2, n_md=4, seed=51, precision=M.F32),
+        S.request((12, 12, 12, 12), M.WILSON, 5.8, n_stout=0, ntraj=3, n_md=4, seed=52, precision=M.F32),
+        S.request((16, 16, 16, 16), M.IWASAKI, 2.4, -0.331, n_stout=2, ntraj=2, n_md=4, seed=53, precision=M.F32),
+        S.request((24, 24, 24, 24), M.WILSON, 6.1, n_stout=1, ntraj=2, n_md=4, seed=54, precision=M.F64),
+        S.request((16, 16, 16, 16), M.IWASAKI, 2.7, -0.331, n_stout=3, ntraj=2, n_md=4, seed=55, precision=M.F64),
+        S.request((12, 12, 12, 12), M.IWASAKI, 2.3, -0.331, n_stout=1, ntraj=3, n_md=4, seed=56, precision=M.F32),
+        S.request((16, 16, 16, 16), M.IWASAKI, 2.45, -0.331, n_stout=2, ntraj=2, n_md=4, seed=57, precision=M.F32),
+    ]
+    alone = [run_alone(ctx, r) for r in reqs]
+    srv = S.Server(ctx, hbm_budget=64 << 30)
+    ids = [srv.submit(r) for r in reqs[:3]]
+    srv.step()
+    ids += [srv.submit(r) for r in reqs[3:5]]
+    srv.step()
+    ids += [srv.submit(r) for r in reqs[5:]]
+    srv.run_until_idle()
+    for cid, a, r in zip(ids, alone, reqs):
+        res, _ = srv.results(cid)
+        assert srv.status(cid) == ("done", r.ntraj)
+        assert key(res) == a
+        assert all(np.isfinite(x.h_new) for x in res)
+    srv.close()
+
+
 def test_server_matches_single_chain_api(ctx):
     req = S.request((8, 8, 8, 8), M.WILSON, 6.0, ntraj=3, n_md=5, seed=21, precision=M.F64)
     served = run_alone(ctx, req)
