@@ -218,8 +218,9 @@ def thermalize(M, ctx, L, arg, seed, tmin):
     ch = M.Chain(ctx, (L,) * 4, dataclasses.replace(arg, num_forced_accept_step=THERM_MAX), seed=seed,
                  precision=M.F32)
     ch.haar(seed)
-    plaq = [r.avg_plaq for r in ch.run(max(tmin, 2 * THERM_BLOCK))]
-    while True:
+    plaq = [r.avg_plaq for r in ch.run(tmin)]
+    drift = None  # below two blocks (profiling runs) the drift rule is not applied
+    while len(plaq) >= 2 * THERM_BLOCK:
         last = sum(plaq[-THERM_BLOCK:]) / THERM_BLOCK
         prev = sum(plaq[-2 * THERM_BLOCK:-THERM_BLOCK]) / THERM_BLOCK
         drift = abs(last - prev) / abs(last)
@@ -227,7 +228,7 @@ def thermalize(M, ctx, L, arg, seed, tmin):
             break
         plaq += [r.avg_plaq for r in ch.run(THERM_BLOCK)]
     ctx.synchronize()
-    print(f"bench: thermalised in {len(plaq)} trajectories, plaquette {plaq[-1]:.6f}, block drift {drift:.1e}",
+    print(f"bench: thermalised in {len(plaq)} trajectories, plaquette {plaq[-1]:.6f}, block drift {drift}",
           file=sys.stderr, flush=True)
     return ch, {"trajectories": len(plaq), "avg_plaq": round(plaq[-1], 8), "block_drift": drift}
 

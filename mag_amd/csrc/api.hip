@@ -43,6 +43,7 @@ void aos_to_soa(const double* h, TS* s, const Geom& g, int kind) {
       for (int mu = 0; mu < 4; ++mu) s[(size_t)mu * g.V + site] = TS(h[(size_t)site * 4 + mu]);
     return;
   }
+  const int w = pack_width_rt((int)sizeof(TS), nc);
   for (int site = 0; site < g.V; ++site)
     for (int mu = 0; mu < 4; ++mu) {
       const double* m = h + ((size_t)site * 4 + mu) * 18;
@@ -51,9 +52,9 @@ void aos_to_soa(const double* h, TS* s, const Geom& g, int kind) {
         for (int k = 0; k < 9; ++k) M.e[k] = {m[2 * k], m[2 * k + 1]};
         double a[8];
         mat_to_alg(M, a);
-        for (int k = 0; k < 8; ++k) s[((size_t)mu * 8 + k) * g.V + site] = TS(a[k]);
+        for (int k = 0; k < 8; ++k) s[elem_index(mu, k, site, 8, w, g.V)] = TS(a[k]);
       } else {
-        for (int k = 0; k < nc; ++k) s[((size_t)mu * nc + k) * g.V + site] = TS(m[k]);
+        for (int k = 0; k < nc; ++k) s[elem_index(mu, k, site, nc, w, g.V)] = TS(m[k]);
       }
     }
 }
@@ -66,18 +67,19 @@ void soa_to_aos(const TS* s, double* h, const Geom& g, int kind) {
       for (int mu = 0; mu < 4; ++mu) h[(size_t)site * 4 + mu] = double(s[(size_t)mu * g.V + site]);
     return;
   }
+  const int w = pack_width_rt((int)sizeof(TS), nc);
   for (int site = 0; site < g.V; ++site)
     for (int mu = 0; mu < 4; ++mu) {
       double* m = h + ((size_t)site * 4 + mu) * 18;
       M3<double> M;
       if (kind == MAG_ALG8) {
         double a[8];
-        for (int k = 0; k < 8; ++k) a[k] = double(s[((size_t)mu * 8 + k) * g.V + site]);
+        for (int k = 0; k < 8; ++k) a[k] = double(s[elem_index(mu, k, site, 8, w, g.V)]);
         M = alg_to_mat(a);
       } else {
         for (int k = 0; k < nc / 2; ++k)
-          M.e[k] = {double(s[((size_t)mu * nc + 2 * k) * g.V + site]),
-                    double(s[((size_t)mu * nc + 2 * k + 1) * g.V + site])};
+          M.e[k] = {double(s[elem_index(mu, 2 * k, site, nc, w, g.V)]),
+                    double(s[elem_index(mu, 2 * k + 1, site, nc, w, g.V)])};
         if (nc == 12) reconstruct_row2(M);
       }
       for (int k = 0; k < 9; ++k) {
