@@ -233,14 +233,15 @@ def thermalize(M, ctx, L, arg, seed, tmin):
     return ch, {"trajectories": len(plaq), "avg_plaq": round(plaq[-1], 8), "block_drift": drift}
 
 
-def timed_chain(M, ctx, L, arg, seed, precision, start, warmup, steps, timing, wloop=False):
+def timed_chain(M, ctx, L, arg, seed, precision, start, warmup, steps, timing, wloop=False, recon12=None):
     """A fresh chain (trajectory 0, Metropolis + demon window of 0.5% of the
     initial action sums) from the links of `start`: `warmup` untimed
     trajectories, then `steps` trajectories timed together between two device
     synchronisations (with `wloop`, one r = (4,0,0), T = 4 Wilson loop after
     each trajectory, inside the timed region). Returns (seconds, warm-up
     results, timed results, chain, Wilson loops)."""
-    ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005)
+    ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005,
+                 recon12=recon12)
     ch.links().copy_from(start.links())
     ch.reset_demon()
     ch.demon_state()  # window centre from the starting links, outside the timed region
@@ -525,6 +526,13 @@ def main():
         # warm-up and timed trajectories of both chains, paired one by one
         out["fp32_vs_fp64"] = paired_acceptance(pre + res, pre64 + r64)
         print(f"bench: FP64 {a.fp64_steps} trajectories in {dt64:.3f} s", file=sys.stderr, flush=True)
+        # the FP64 path with links stored as 12 reals (row 2 rebuilt on load)
+        dt12, _, r12, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_steps, False,
+                                         a.wloop, recon12=True)
+        out["fp64_12real_traj_per_s"] = round(a.fp64_steps / dt12, 4)
+        out["fp32_over_fp64_12real"] = round(traj_s / (a.fp64_steps / dt12), 3)
+        out["fp64_12real_accept_rate"] = round(sum(r.accept for r in r12) / len(r12), 3)
+        print(f"bench: FP64 12-real {a.fp64_steps} trajectories in {dt12:.3f} s", file=sys.stderr, flush=True)
     if a.reversibility:
         dev, dh_rt = ch32.reversibility()
         out["reversibility_fp64"] = {"max_link_dev": dev, "dh_roundtrip": dh_rt}

@@ -79,7 +79,8 @@ typedef struct {
   uint64_t seed;           /* Philox key of the chain */
   int reunitarize;         /* 1: reunitarize after every trajectory */
   int precision;           /* MAG_F64: all FP64; MAG_F32: FP32 MD, FP64 H */
-  int recon12;             /* FP32 links stored as 12 reals */
+  int recon12;             /* FP64 links stored as 12 reals (rows 0, 1; row 2 rebuilt on load);
+                              FP32 links are always 12-real */
 } mag_hmc_params;
 
 typedef struct {

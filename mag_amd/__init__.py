@@ -422,13 +422,15 @@ class Chain:
     """Device-resident HMC chain (run_chmc / update_field, field-hmc.h:770-986)."""
 
     def __init__(self, ctx, dims, arg, seed=1234, precision=F64, integrator=0, accept_mode=0, demon_emax=0.0,
-                 reunit=True):
+                 reunit=True, recon12=None):
+        """recon12: FP64 links stored as 12 reals (FP32 links always are)."""
         self.ctx = ctx
         self.dims = tuple(dims)
         self.arg = arg
         self.act = _act(arg)
+        r12 = precision == F32 or bool(recon12)
         self.params = HmcParams(arg.trajectory_length, arg.dt, integrator, arg.num_forced_accept_step, accept_mode,
-                                demon_emax, seed, 1 if reunit else 0, precision, 1 if precision == F32 else 0)
+                                demon_emax, seed, 1 if reunit else 0, precision, 1 if r12 else 0)
         self._h = C.c_void_p()
         _check(_lib.lib().mag_chain_create(ctx.handle, (C.c_int * 4)(*self.dims), C.byref(self.act),
                                            C.byref(self.params), C.byref(self._h)), ctx.handle)
@@ -460,7 +462,7 @@ class Chain:
     def links(self):
         h = C.c_void_p()
         _check(_lib.lib().mag_chain_links(self._h, C.byref(h)), self.ctx.handle)
-        kind = LINK12 if self.params.precision == F32 else LINK18
+        kind = LINK12 if self.params.recon12 else LINK18
         return Field(self.ctx, self.dims, kind, self.params.precision, _handle=h)
 
     def reset_demon(self):

@@ -212,8 +212,8 @@ int mag_field_create(mag_ctx c, const int dims[4], int kind, int prec, mag_field
   if (kind < MAG_LINK18 || kind > MAG_REAL1 || (prec != MAG_F32 && prec != MAG_F64))
     return fail(c, MAG_ERR_ARG, "bad field kind or precision");
   if (kind == MAG_REAL1 && prec != MAG_F64) return fail(c, MAG_ERR_UNSUPPORTED, "real-per-link fields are FP64");
-  if ((kind == MAG_LINK12 && prec == MAG_F64) || (kind == MAG_LINK18 && prec == MAG_F32))
-    return fail(c, MAG_ERR_UNSUPPORTED, "link fields are FP64 18-real or FP32 12-real");
+  if (kind == MAG_LINK18 && prec == MAG_F32)
+    return fail(c, MAG_ERR_UNSUPPORTED, "link fields are FP64 18- or 12-real, or FP32 12-real");
   auto* f = new mag_field_s();
   f->ctx = c;
   std::memcpy(f->dims, dims, sizeof f->dims);

@@ -160,7 +160,7 @@ int ctx_child_create(mag_ctx parent, hipStream_t stream, std::shared_ptr<MemPool
 size_t chain_footprint(const int dims[4], const mag_action& act, const mag_hmc_params& p) {
   const size_t V = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
   const bool f32 = p.precision == MAG_F32;
-  const size_t rs = f32 ? 4 : 8, nc = f32 ? 12 : 18;
+  const size_t rs = f32 ? 4 : 8, nc = (f32 || p.recon12) ? 12 : 18;
   const size_t link = V * 4 * nc * rs, alg = V * 4 * 8 * rs, mat = V * 4 * 18 * rs, mat64 = V * 4 * 18 * 8;
   size_t b = 3 * link + 2 * alg;  // U, Ubak, Uaux; P, F
   if (act.n_stout > 0) {
@@ -191,7 +191,8 @@ int mag_chain_create(mag_ctx ctx, const int dims[4], const mag_action* act, cons
   c->act = *act;
   c->p = *p;
   c->prec = p->precision == MAG_F32 ? MAG_F32 : MAG_F64;
-  c->kind = c->prec == MAG_F32 ? MAG_LINK12 : MAG_LINK18;
+  // FP32 links are always 12-real; FP64 links 12-real when recon12 is set
+  c->kind = (c->prec == MAG_F32 || p->recon12) ? MAG_LINK12 : MAG_LINK18;
   int r;
   if ((r = mag_field_create(ctx, dims, c->kind, c->prec, &c->U)) ||
       (r = mag_field_create(ctx, dims, c->kind, c->prec, &c->Ubak)) ||

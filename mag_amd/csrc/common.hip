@@ -64,22 +64,22 @@ int alg_scale(mag_ctx c, const Geom& g, int prec, void* P, double s) {
   return MAG_OK;
 }
 
+// Link format conversion between FP64 18-real, FP64 12-real and FP32 12-real
+// (through FP64 arithmetic; 12-real outputs keep rows 0 and 1).
 int convert_links(mag_ctx c, const Geom& g, int prec_in, int kind_in, void* in, int prec_out, int kind_out,
                   void* out) {
-  const bool in64 = prec_in == MAG_F64 && kind_in == MAG_LINK18;
-  const bool in32 = prec_in == MAG_F32 && kind_in == MAG_LINK12;
-  const bool out64 = prec_out == MAG_F64 && kind_out == MAG_LINK18;
-  const bool out32 = prec_out == MAG_F32 && kind_out == MAG_LINK12;
-  if (in64 && out32)
-    hipLaunchKernelGGL((k_convert_links<double, 18, float, 12>), grid_links(g), dim3(BLOCK), 0, c->stream,
-                       LinkView<double, 18>{(double*)in, g.V}, LinkView<float, 12>{(float*)out, g.V}, g.V);
-  else if (in32 && out64)
-    hipLaunchKernelGGL((k_convert_links<float, 12, double, 18>), grid_links(g), dim3(BLOCK), 0, c->stream,
-                       LinkView<float, 12>{(float*)in, g.V}, LinkView<double, 18>{(double*)out, g.V}, g.V);
-  else
-    return fail(c, MAG_ERR_UNSUPPORTED, "unsupported link conversion");
-  CHECK_LAUNCH(c);
-  return MAG_OK;
+  return dispatch_link(c, prec_in, kind_in, [&](auto ti) -> int {
+    using TI = typename decltype(ti)::T;
+    constexpr int NI = decltype(ti)::N;
+    return dispatch_link(c, prec_out, kind_out, [&](auto to) -> int {
+      using TO = typename decltype(to)::T;
+      constexpr int NO = decltype(to)::N;
+      hipLaunchKernelGGL((k_convert_links<TI, NI, TO, NO>), grid_links(g), dim3(BLOCK), 0, c->stream,
+                         LinkView<TI, NI>{(TI*)in, g.V}, LinkView<TO, NO>{(TO*)out, g.V}, g.V);
+      CHECK_LAUNCH(c);
+      return MAG_OK;
+    });
+  });
 }
 
 int batch_expiQ(mag_ctx c, const double* dQ, double* dout, long n, int prec) {

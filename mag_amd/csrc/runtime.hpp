@@ -349,7 +349,8 @@ template <class F>
 int dispatch_link(mag_ctx c, int prec, int kind, F&& f) {
   if (prec == MAG_F64 && kind == MAG_LINK18) return f(Tag<double, 18>{});
   if (prec == MAG_F32 && kind == MAG_LINK12) return f(Tag<float, 12>{});
-  return fail(c, MAG_ERR_UNSUPPORTED, "link fields are FP64 18-real or FP32 12-real");
+  if (prec == MAG_F64 && kind == MAG_LINK12) return f(Tag<double, 12>{});
+  return fail(c, MAG_ERR_UNSUPPORTED, "link fields are FP64 18- or 12-real, or FP32 12-real");
 }
 
 // ------------------------------------------------------------ common ops

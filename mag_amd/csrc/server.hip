@@ -75,7 +75,8 @@ struct Entry {
 using ClassKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
 
 ClassKey class_of(const mag_serve_request& r) {
-  return ClassKey(r.dims[0], r.dims[1], r.dims[2], r.dims[3], r.hmc.precision == MAG_F32 ? 1 : 0, r.act.type,
+  const int fmt = r.hmc.precision == MAG_F32 ? 1 : (r.hmc.recon12 ? 2 : 0);  // link storage format
+  return ClassKey(r.dims[0], r.dims[1], r.dims[2], r.dims[3], fmt, r.act.type,
                   r.act.n_stout, r.act.mag, r.hmc.integrator, r.hmc.trajectory_length, r.hmc.reunitarize ? 1 : 0);
 }
 

@@ -25,10 +25,10 @@ def _check(code, ctx):
 
 
 def request(dims, act_type=WILSON, beta=6.0, c1=0.0, n_stout=0, rho=0.1, ntraj=5, n_md=10, tau=1.0, seed=1,
-            precision=F32, links_seed=None, accept_mode=0, demon_emax=0.0, num_forced_accept=0):
+            precision=F32, links_seed=None, accept_mode=0, demon_emax=0.0, num_forced_accept=0, recon12=False):
     a = Action(act_type, beta, c1 if act_type == IWASAKI else 0.0, 0, n_stout, rho)
     h = HmcParams(n_md, tau / n_md, 0, num_forced_accept, accept_mode, demon_emax, seed, 1, precision,
-                  1 if precision == F32 else 0)
+                  1 if (precision == F32 or recon12) else 0)
     return ServeRequest((C.c_int * 4)(*dims), a, h, ntraj, seed if links_seed is None else links_seed)
 
 

@@ -250,6 +250,53 @@ def test_fg_trajectory_c1(ctx, orc):
     assert dev < 1e-12 and dev_ref < 1e-12
 
 
+# FP64 links stored as 12 reals (row 2 rebuilt on load): the same parity bar
+# as the 18-real FP64 path on exact SU(3) (Haar) links.
+@pytest.mark.parametrize("name", ["wilson", "iwasaki", "iwasaki_stout3", "iwasaki_mag2"])
+def test_force_fp64_12real(ctx, orc, name):
+    a = orc.action(**ACTIONS[name])
+    L = L8 if a.n_stout == 0 else L4
+    U = orc.haar_links(L, 17)
+    ref = orc.get_force(L, U, a)
+    F = alg(ctx, L)
+    U12 = M.Field(ctx, L, LINK12, F64).upload(U)
+    assert rel(U12.download(), U) < 1e-13  # row 2 rebuilt as conj(row0 x row1)
+    M.get_force(F, U12, M._lib.Action(a.type, a.beta, a.c1, a.mag, a.n_stout, a.rho))
+    assert rel(F.download(), ref) < 1e-12
+
+
+def test_fg_trajectory_stout_fp64_12real(ctx, orc):
+    """Iwasaki + 3-stout FG trajectory with reunitarization on FP64 12-real links, and its Hamiltonian."""
+    a = orc.action(1, 2.6, -0.331, n_stout=3, rho=0.1)
+    U, P = orc.haar_links(L4, 38), orc.init_momentum(L4, 38, 0)
+    Ur, Pr = U.copy(), P.copy()
+    orc.force_gradient_integrator(L4, Ur, Pr, a, 4, 0.1, reunit=True)
+    h_ref, _ = orc.hamiltonian(L4, Ur, Pr, a)
+    dU, dP = M.Field(ctx, L4, LINK12, F64).upload(U), alg(ctx, L4, P)
+    arg = M.Arg_chmc(trajectory_length=4, dt=0.1, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=3, rho=0.1)
+    M.force_gradient_integrator(dU, dP, arg)
+    assert rel(dU.download(), Ur) < 1e-10
+    assert rel(dP.download(), Pr) < 1e-10
+    h, _ = M.get_hamiltonian(dU, dP, arg)
+    assert abs(h - h_ref) < 1e-9 * abs(h_ref)
+
+
+def test_accept_sequence_fp64_12real(ctx, orc):
+    """The 20-trajectory accept sequence of the 18-real FP64 chain, with 12-real FP64 links."""
+    ref, res18, _, _ = _chain_vs_oracle(ctx, orc, L4, dict(type=1, beta=2.6, c1=-0.331, n_stout=1, rho=0.1),
+                                        dict(trajectory_length=4, dt=0.25, num_forced_accept=3, seed=98), 20)
+    arg = M.Arg_chmc(trajectory_length=4, dt=0.25, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=1, rho=0.1,
+                     num_forced_accept_step=3)
+    ch = M.Chain(ctx, L4, arg, seed=98, precision=F64, recon12=True)
+    ch.set_links(orc.haar_links(L4, 1234))
+    res = ch.run(20)
+    assert [r.accept for r in res] == [r.accept for r in ref] == [r.accept for r in res18]
+    assert 0 < sum(r.accept for r in ref[3:]) < 17
+    for r, q in zip(res, ref):
+        assert abs(r.h_old - q.h_old) < 1e-12 * abs(q.h_old)
+        assert abs(r.dh - q.dh) < 1e-8 * max(1, abs(q.dh))
+
+
 def test_fg_trajectory_stout_fp64(ctx, orc):
     a = orc.action(1, 2.6, -0.331, n_stout=2, rho=0.1)
     U, P = orc.haar_links(L4, 37), orc.init_momentum(L4, 37, 0)
