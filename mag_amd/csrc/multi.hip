@@ -103,8 +103,8 @@ MAG_D void for_nu(F&& f) {
   if constexpr (MU != 3) f(IC<3>{});
 }
 // Directions are compile-time inside every kernel body: the block-uniform
-// blockIdx.y selects a specialised body, so shifts and field-plane offsets
-// fold to constants.
+// row of the work tile (ctile) selects a specialised body, so shifts and
+// field-plane offsets fold to constants.
 template <class F>
 MAG_D void with_mu(int mu, F&& f) {
   switch (mu) {
@@ -195,6 +195,26 @@ MAG_D void coarse_Q(const MultiArgs& a, const int* x0, D q[8], M3<D>& U2) {
   for (int k = 0; k < 8; ++k) q[k] *= a.rho;
 }
 
+// Work tile of a 1D grid of nb * ny blocks: nb blocks of 256 coarse sites
+// times ny rows (directions, or (mu, nu) pairs). Blocks go to the 8 XCDs
+// round-robin; when nb splits into 8, XCD k owns the coarse blocks
+// [k nb/8, (k+1) nb/8) (a slab of x3, which is slowest in the site order)
+// and walks them with the row fastest, so the rows of one site block run
+// back to back on one XCD and the fine links they share come into its L2
+// once; the slab's neighbours along x3 are on at most two other XCDs.
+// Otherwise row-major order.
+struct CTile {
+  int row, blk;
+};
+MAG_D CTile ctile(const MultiArgs& a) {
+  const int nb = (a.gc.V + BLOCK - 1) / BLOCK, ny = gridDim.x / nb, b = blockIdx.x;
+  if (nb % 8 == 0) {
+    const int j = b >> 3;
+    return {j % ny, (b & 7) * (nb >> 3) + j / ny};
+  }
+  return {b / nb, b % nb};
+}
+
 MAG_D void coarse_site(const MultiArgs& a, int sc, int* x0) {
   const Site pc = site_of(a.gc, sc);
 #pragma unroll
@@ -203,11 +223,12 @@ MAG_D void coarse_site(const MultiArgs& a, int sc, int* x0) {
 
 // Lambda (compute_Lambda with Sigma' = C^+, field-multi-md.h:164-176) and e^{iQ}.
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_prep(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x0[4];
   coarse_site(a, sc, x0);
-  with_mu(blockIdx.y, [&](auto mc) {
+  with_mu(tl.row, [&](auto mc) {
     constexpr int MU = decltype(mc)::value;
     D q[8];
     M3<D> U2;
@@ -376,12 +397,13 @@ MAG_D void add_stout_term(const MultiArgs& a, const int* x, const M3<D>& t) {
 // directions other than mu).
 template <int CLS>
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_fforce(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x[4];
   coarse_site(a, sc, x);
   if constexpr (CLS == 1 || CLS == 3) {
-    with_mu(blockIdx.y, [&](auto mc) {
+    with_mu(tl.row, [&](auto mc) {
       constexpr int MU = decltype(mc)::value;
       if constexpr (CLS == 1) {
         add_stout_term<MU>(a, x, fclass1<MU>(a, x, sc));
@@ -391,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_fforce(MultiArgs 
       }
     });
   } else {
-    with_mu_nu(blockIdx.y, [&](auto mc, auto nc) {
+    with_mu_nu(tl.row, [&](auto mc, auto nc) {
       constexpr int MU = decltype(mc)::value, NU = decltype(nc)::value;
       shift<NU>(a.gf, x, 1);
       if constexpr (CLS == 2) {
@@ -526,7 +548,8 @@ MAG_D void cforce_link(const MultiArgs& a, const int* x0, int sc) {
 // one kernel per direction (its own register budget and stack frame)
 template <int MU>
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_cforce(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x0[4];
   coarse_site(a, sc, x0);
@@ -543,11 +566,12 @@ MAG_D M3<D> coarse_Gb(const MultiArgs& a, const int* x0) {
 }
 
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_gb(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x0[4];
   coarse_site(a, sc, x0);
-  with_mu(blockIdx.y, [&](auto mc) {
+  with_mu(tl.row, [&](auto mc) {
     constexpr int MU = decltype(mc)::value;
     a.out.store(MU, sc, coarse_Gb<MU>(a, x0));
   });
@@ -555,11 +579,12 @@ __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_gb(MultiArgs a) {
 
 // get_Q (field-multi.h:33-39) per coarse link, stored as its 8 generator components
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_getq(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x0[4];
   coarse_site(a, sc, x0);
-  with_mu(blockIdx.y, [&](auto mc) {
+  with_mu(tl.row, [&](auto mc) {
     constexpr int MU = decltype(mc)::value;
     D q[8];
     M3<D> U2;
@@ -571,18 +596,19 @@ __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_getq(MultiArgs a)
 // per-link terms xi Re Tr(C^+ Gb), one direction per grid row (summed by the
 // final reduction over the partial sums of all rows)
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_xi(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   double acc = 0.;
   if (sc < a.gc.V) {
     int x0[4];
     coarse_site(a, sc, x0);
-    with_mu(blockIdx.y, [&](auto mc) {
+    with_mu(tl.row, [&](auto mc) {
       constexpr int MU = decltype(mc)::value;
       acc = retrace_mul_nd(coarse_Gb<MU>(a, x0), a.C.template load<D>(MU, sc)) * xi_at(a, MU, sc);
     });
   }
   const double t = block_sum(acc);
-  if (threadIdx.x == 0) a.part[blockIdx.y * gridDim.x + blockIdx.x] = t;
+  if (threadIdx.x == 0) a.part[tl.row * ((a.gc.V + BLOCK - 1) / BLOCK) + tl.blk] = t;
 }
 
 // simplest_metropolis with random_su3_from_su2(0.9) (field-multi-md.h:748-769,
@@ -615,11 +641,12 @@ MAG_D void heatbath_link(const MultiArgs& a, const int* x0, int sc) {
 }
 
 __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_heatbath(MultiArgs a) {
-  const int sc = blockIdx.x * BLOCK + threadIdx.x;
+  const CTile tl = ctile(a);
+  const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x0[4];
   coarse_site(a, sc, x0);
-  with_mu(blockIdx.y, [&](auto mc) { heatbath_link<decltype(mc)::value>(a, x0, sc); });
+  with_mu(tl.row, [&](auto mc) { heatbath_link<decltype(mc)::value>(a, x0, sc); });
 }
 
 // ---------------------------------------------------------------- host side
@@ -693,7 +720,8 @@ int mlaunch(const Multi& m, const char* name, void (*k)(MultiArgs), dim3 grid, c
   return MAG_OK;
 }
 
-dim3 cgrid(const Multi& m, int y) { return dim3((m.gc.V + BLOCK - 1) / BLOCK, y, 1); }
+// 1D grid of (coarse site block, row) tiles; see ctile()
+dim3 cgrid(const Multi& m, int y) { return dim3((unsigned)(((m.gc.V + BLOCK - 1) / BLOCK) * y), 1, 1); }
 
 // md::get_Fforce into F (fine alg8)
 int fforce(const Multi& m, void* F, void* U, void* C, const void* xi, const mag_action& act) {
@@ -736,7 +764,7 @@ int xi_sum(const Multi& m, void* U, void* C, const void* xi, int slot) {
   if ((r = ensure_part(m.c, 4 * (size_t)nb))) return r;
   MultiArgs a = args(m, U, C, xi);
   a.part = m.c->d_part;
-  if ((r = mlaunch(m, "multi_xi", k_multi_xi, dim3(nb, 4, 1), a))) return r;
+  if ((r = mlaunch(m, "multi_xi", k_multi_xi, cgrid(m, 4), a))) return r;
   return final_reduce(m.c, m.c->d_part, 4 * nb, 1, 0, slot);
 }
 
