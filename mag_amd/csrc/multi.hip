@@ -52,6 +52,15 @@ using AV = AlgView<double>;
 #define MAG_MULTI_WAVES 1
 #endif
 #define MAG_MULTI_OCC __attribute__((amdgpu_waves_per_eu(MAG_MULTI_WAVES)))
+// occupancy floors of the fine-force kernels (heavy classes 1 and 3) and the coarse force
+#ifndef MAG_FFORCE_WAVES
+#define MAG_FFORCE_WAVES 1
+#endif
+#ifndef MAG_CFORCE_WAVES
+#define MAG_CFORCE_WAVES 1
+#endif
+#define MAG_FFORCE_OCC __attribute__((amdgpu_waves_per_eu(MAG_FFORCE_WAVES)))
+#define MAG_CFORCE_OCC __attribute__((amdgpu_waves_per_eu(MAG_CFORCE_WAVES)))
 
 struct MultiArgs {
   Geom gf, gc;
@@ -396,7 +405,7 @@ MAG_D void add_stout_term(const MultiArgs& a, const int* x, const M3<D>& t) {
 // grid.y: class 1 and 3: mu; class 2 and 4: 3 mu + (index of nu among the
 // directions other than mu).
 template <int CLS>
-__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_fforce(MultiArgs a) {
+__global__ __launch_bounds__(BLOCK) MAG_FFORCE_OCC void k_multi_fforce(MultiArgs a) {
   const CTile tl = ctile(a);
   const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
@@ -547,7 +556,7 @@ MAG_D void cforce_link(const MultiArgs& a, const int* x0, int sc) {
 
 // one kernel per direction (its own register budget and stack frame)
 template <int MU>
-__global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_cforce(MultiArgs a) {
+__global__ __launch_bounds__(BLOCK) MAG_CFORCE_OCC void k_multi_cforce(MultiArgs a) {
   const CTile tl = ctile(a);
   const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
