@@ -79,6 +79,8 @@ def parse():
     p.add_argument("--n-stout", type=int, default=3)
     p.add_argument("--fp64-steps", type=int, default=-1,
                    help="trajectories of the FP64 comparison chain (default: as many as --steps; 0: skip)")
+    p.add_argument("--fp64-12-steps", type=int, default=-1,
+                   help="trajectories of the FP64 chain with 12-real links (default: as --fp64-steps; 0: skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--serve-chains", type=int, default=512,
                    help="chains of the serving stream (0: skip); the full stream has 512")
@@ -178,12 +180,13 @@ def serve_stream(M, ctx, nchains, max_traj):
     stream = S.poisson_stream(nchains=nchains, rate=200.0, seed=2026, traj_range=(5, max_traj))
     srv = S.Server(ctx, hbm_budget=200 << 30, max_batch=64, graphs=True)
     out, wall = S.drive(srv, stream, log=lambda m: print("bench: serving " + m, file=sys.stderr, flush=True))
-    st = S.latency_stats(out)
+    st = S.latency_stats(out, srv.extent)
     srv.close()
     total = sum(r.ntraj for _, r in stream)
     st.update({"chains": nchains, "wall_s": round(wall, 3), "traj_per_s": round(st["trajectories"] / wall, 2),
                "sample": (f"the full {nchains}-chain stream, {total} trajectories" if nchains == 512 and max_traj == 50
                           else f"first {nchains} chains of the 512-chain stream, trajectories 5..{max_traj}")})
+    st["traj_latency_ms_by_extent_note"] = "p50, p99 of each lattice extent's per-trajectory gaps"
     return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
 
 
@@ -444,6 +447,8 @@ def main():
         return
     if a.fp64_steps < 0:
         a.fp64_steps = a.steps
+    if a.fp64_12_steps < 0:
+        a.fp64_12_steps = a.fp64_steps
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)", file=sys.stderr)
         sys.exit(2)
@@ -526,13 +531,14 @@ def main():
         # warm-up and timed trajectories of both chains, paired one by one
         out["fp32_vs_fp64"] = paired_acceptance(pre + res, pre64 + r64)
         print(f"bench: FP64 {a.fp64_steps} trajectories in {dt64:.3f} s", file=sys.stderr, flush=True)
+    if a.fp64_12_steps > 0:
         # the FP64 path with links stored as 12 reals (row 2 rebuilt on load)
-        dt12, _, r12, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_steps, False,
+        dt12, _, r12, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_12_steps, False,
                                          a.wloop, recon12=True)
-        out["fp64_12real_traj_per_s"] = round(a.fp64_steps / dt12, 4)
-        out["fp32_over_fp64_12real"] = round(traj_s / (a.fp64_steps / dt12), 3)
+        out["fp64_12real_traj_per_s"] = round(a.fp64_12_steps / dt12, 4)
+        out["fp32_over_fp64_12real"] = round(traj_s / (a.fp64_12_steps / dt12), 3)
         out["fp64_12real_accept_rate"] = round(sum(r.accept for r in r12) / len(r12), 3)
-        print(f"bench: FP64 12-real {a.fp64_steps} trajectories in {dt12:.3f} s", file=sys.stderr, flush=True)
+        print(f"bench: FP64 12-real {a.fp64_12_steps} trajectories in {dt12:.3f} s", file=sys.stderr, flush=True)
     if a.reversibility:
         dev, dh_rt = ch32.reversibility()
         out["reversibility_fp64"] = {"max_link_dev": dev, "dh_roundtrip": dh_rt}

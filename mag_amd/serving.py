@@ -132,6 +132,7 @@ def drive(server, stream, time_scale=1.0, log=None, log_every=20.0):
     t0 = time.perf_counter()
     info0 = server.info()[2]
     ids, k = {}, 0
+    server.extent = {}
     next_log = log_every
     while True:
         now = time.perf_counter() - t0
@@ -140,7 +141,9 @@ def drive(server, stream, time_scale=1.0, log=None, log_every=20.0):
             log(f"{now:.0f} s: {k}/{len(stream)} submitted, {ntr} trajectories done, {u / 2**30:.1f} GiB in use")
             next_log += log_every
         while k < len(stream) and stream[k][0] * time_scale <= now:
-            ids[server.submit(stream[k][1])] = stream[k][0] * time_scale
+            cid = server.submit(stream[k][1])
+            ids[cid] = stream[k][0] * time_scale
+            server.extent[cid] = stream[k][1].dims[0]
             k += 1
         a, p = server.step()
         if a == 0 and p == 0:
@@ -154,20 +157,30 @@ def drive(server, stream, time_scale=1.0, log=None, log_every=20.0):
     return out, time.perf_counter() - t0
 
 
-def latency_stats(out):
+def latency_stats(out, extent=None):
     """Per-trajectory latency = gap between a chain's successive completions
-    (the first measured from its arrival); chain latency = arrival -> done."""
+    (the first measured from its arrival); chain latency = arrival -> done.
+    With extent (chain id -> lattice extent), also the p50/p99 per-trajectory
+    latency of each extent."""
     gaps, chain_lat, ntraj = [], [], 0
-    for arr, t, res in out.values():
+    by = {}
+    for cid, (arr, t, res) in out.items():
         prev = arr
         for x in t:
             gaps.append(x - prev)
+            if extent is not None:
+                by.setdefault(extent[cid], []).append(x - prev)
             prev = x
         if t:
             chain_lat.append(t[-1] - arr)
         ntraj += len(t)
     g = np.array(gaps)
-    return {"trajectories": ntraj, "p50_traj_latency_ms": float(np.percentile(g, 50) * 1e3),
-            "p99_traj_latency_ms": float(np.percentile(g, 99) * 1e3),
-            "p50_chain_latency_s": float(np.percentile(chain_lat, 50)),
-            "p99_chain_latency_s": float(np.percentile(chain_lat, 99))}
+    st = {"trajectories": ntraj, "p50_traj_latency_ms": float(np.percentile(g, 50) * 1e3),
+          "p99_traj_latency_ms": float(np.percentile(g, 99) * 1e3),
+          "p50_chain_latency_s": float(np.percentile(chain_lat, 50)),
+          "p99_chain_latency_s": float(np.percentile(chain_lat, 99))}
+    if extent is not None:
+        st["traj_latency_ms_by_extent"] = {
+            f"{L}^4": [round(float(np.percentile(v, 50) * 1e3), 1), round(float(np.percentile(v, 99) * 1e3), 1)]
+            for L, v in sorted(by.items())}
+    return st
