@@ -97,6 +97,8 @@ def parse():
                    help="time the coupled coarse/fine md:: HMC (FP64) instead of the headline workload")
     p.add_argument("--no-reversibility", dest="reversibility", action="store_false",
                    help="skip the FP64 reversibility round trip after the timed run")
+    p.add_argument("--fp64-kernel-times", action="store_true",
+                   help="also time the kernels of the FP64 chains (HIP events around each launch)")
     return p.parse_args()
 
 
@@ -266,6 +268,16 @@ def timed_chain(M, ctx, L, arg, seed, precision, start, warmup, steps, timing, w
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
     return dt, pre, res, ch, loops
+
+
+def kernel_ms(ctx, steps):
+    """HIP-event kernel time per trajectory of every timed kernel family."""
+    out = {}
+    for f in list(KERNEL_UNITS) + ["stout_fwd_h", "action", "kinetic", "reduce", "mom_refresh", "reunitarize"]:
+        ms, nl = ctx.kernel_time(f)
+        if nl:
+            out[f] = round(ms / steps, 3)
+    return out
 
 
 def paired_acceptance(r32, r64):
@@ -523,8 +535,10 @@ def main():
     if out["accept_rate"] == 0:
         out["invalid"] = "the measured chain accepted no trajectory"
     if a.fp64_steps > 0:
-        dt64, pre64, r64, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_steps, False,
-                                             a.wloop)
+        dt64, pre64, r64, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_steps,
+                                             a.fp64_kernel_times, a.wloop)
+        if a.fp64_kernel_times:
+            out["fp64_kernel_ms_per_traj"] = kernel_ms(ctx, a.fp64_steps)
         out["fp64_traj_per_s"] = round(a.fp64_steps / dt64, 4)
         out["fp32_over_fp64"] = round(traj_s / (a.fp64_steps / dt64), 3)
         out["fp64_accept_rate"] = round(sum(r.accept for r in r64) / len(r64), 3)
@@ -533,8 +547,10 @@ def main():
         print(f"bench: FP64 {a.fp64_steps} trajectories in {dt64:.3f} s", file=sys.stderr, flush=True)
     if a.fp64_12_steps > 0:
         # the FP64 path with links stored as 12 reals (row 2 rebuilt on load)
-        dt12, _, r12, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_12_steps, False,
-                                         a.wloop, recon12=True)
+        dt12, _, r12, _, _ = timed_chain(M, ctx, L, arg, a.seed, M.F64, therm, a.warmup, a.fp64_12_steps,
+                                         a.fp64_kernel_times, a.wloop, recon12=True)
+        if a.fp64_kernel_times:
+            out["fp64_12real_kernel_ms_per_traj"] = kernel_ms(ctx, a.fp64_12_steps)
         out["fp64_12real_traj_per_s"] = round(a.fp64_12_steps / dt12, 4)
         out["fp32_over_fp64_12real"] = round(traj_s / (a.fp64_12_steps / dt12), 3)
         out["fp64_12real_accept_rate"] = round(sum(r.accept for r in r12) / len(r12), 3)

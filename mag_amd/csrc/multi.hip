@@ -80,6 +80,23 @@ struct MultiArgs {
 template <int K>
 using IC = std::integral_constant<int, K>;
 
+// A stored SU(3) link (fine or coarse link, e^{iQ}). MAG_MD_READ12 = 1
+// reads rows 0 and 1 (6 of the 9 16-byte groups) and rebuilds row 2 as
+// conj(r0 x r1): these kernels are bound by the vector-memory path, with the
+// vector ALUs mostly idle.
+#ifndef MAG_MD_READ12
+#define MAG_MD_READ12 1
+#endif
+MAG_D M3<D> ld(const LV& v, int mu, int s) {
+  if constexpr (MAG_MD_READ12) {
+    M3<D> m = v.template load_r2<D>(mu, s);
+    reconstruct_row2(m);
+    return m;
+  } else {
+    return v.template load<D>(mu, s);
+  }
+}
+
 // Shift of a fine coordinate along the compile-time direction K, wrapped
 // into [0, L) (|n| <= 2 < L).
 template <int K>
@@ -161,13 +178,13 @@ struct Walk {
   }
   template <int K>
   MAG_D void fwd() {
-    right(u.template load<D>(K, lin(g, y)));
+    right(ld(u, K, lin(g, y)));
     shift<K>(g, y, 1);
   }
   template <int K>
   MAG_D void bwd() {
     shift<K>(g, y, -1);
-    const M3<D> a = u.template load<D>(K, lin(g, y));
+    const M3<D> a = ld(u, K, lin(g, y));
     if (fresh) m = dag(a);
     else m = mul_nd(m, a);
     fresh = false;
@@ -182,9 +199,9 @@ MAG_D void acc_i(M3<D>& t, double w, const M3<D>& m) { caxpy(t, cpx<D>{0., w}, m
 template <int MU>
 MAG_D void coarse_Q(const MultiArgs& a, const int* x0, D q[8], M3<D>& U2) {
   int y[4] = {x0[0], x0[1], x0[2], x0[3]};
-  const M3<D> U1 = a.U.template load<D>(MU, lin(a.gf, y));
+  const M3<D> U1 = ld(a.U, MU, lin(a.gf, y));
   shift<MU>(a.gf, y, 1);
-  U2 = mul(U1, a.U.template load<D>(MU, lin(a.gf, y)));
+  U2 = mul(U1, ld(a.U, MU, lin(a.gf, y)));
   M3<D> stp = zero3<D>();
   for_nu<MU>([&](auto nc) {
     constexpr int NU = decltype(nc)::value;
@@ -242,7 +259,7 @@ __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_prep(MultiArgs a)
     D q[8];
     M3<D> U2;
     coarse_Q<MU>(a, x0, q, U2);
-    const M3<D> X = mul_nd(U2, a.C.template load<D>(MU, sc));  // U2 Sigma'
+    const M3<D> X = mul_nd(U2, ld(a.C, MU, sc));  // U2 Sigma'
     D lam[8];
     M3<D> eQ;
     ch_lambda(alg_to_mat(q), X, lam, eQ);
@@ -264,8 +281,8 @@ MAG_D M3<D> fclass1(const MultiArgs& a, const int* x, int sc) {
   const D xm = xi_at(a, MU, sc);
   const M3<D> Lm = a.lam.template load_mat<D>(MU, sc);
   // direct term U(x+mu) C^+ e^{iQ} xi
-  M3<D> t = scale(mul(mul_nd(a.U.template load<D>(MU, lin(a.gf, s)), a.C.template load<D>(MU, sc)),
-                      a.eq.template load<D>(MU, sc)),
+  M3<D> t = scale(mul(mul_nd(ld(a.U, MU, lin(a.gf, s)), ld(a.C, MU, sc)),
+                      ld(a.eq, MU, sc)),
                   xm);
   for_nu<MU>([&](auto nc) {
     constexpr int NU = decltype(nc)::value;
@@ -329,9 +346,9 @@ MAG_D M3<D> fclass3(const MultiArgs& a, const int* x, int sc) {
   shift<MU>(a.gf, ym, -1);  // = 2c
   const D xm = xi_at(a, MU, sc);
   const M3<D> Lm = a.lam.template load_mat<D>(MU, sc);
-  const M3<D> Uym = a.U.template load<D>(MU, lin(a.gf, ym));
+  const M3<D> Uym = ld(a.U, MU, lin(a.gf, ym));
   // direct term C^+ e^{iQ} U(x-mu) xi
-  M3<D> t = scale(mul(mul_dn(a.C.template load<D>(MU, sc), a.eq.template load<D>(MU, sc)), Uym), xm);
+  M3<D> t = scale(mul(mul_dn(ld(a.C, MU, sc), ld(a.eq, MU, sc)), Uym), xm);
   int yp[4] = {x[0], x[1], x[2], x[3]};
   shift<MU>(a.gf, yp, 1);  // = 2(c + mu)
   const int cp = cidx(a.gc, yp);
@@ -393,7 +410,7 @@ template <int MU>
 MAG_D void add_stout_term(const MultiArgs& a, const int* x, const M3<D>& t) {
   const int s = lin(a.gf, x);
   D w[8], f[8];
-  force_alg_from(mul(a.U.template load<D>(MU, s), t), w);
+  force_alg_from(mul(ld(a.U, MU, s), t), w);
   a.F.load(MU, s, f);
 #pragma unroll
   for (int k = 0; k < 8; ++k) f[k] -= w[k];
@@ -455,7 +472,7 @@ struct DWalk {
   }
   MAG_D void fwd(int K) {
     const int s = lin(g, y);
-    const M3<D> X = u.template load<D>(K, s);
+    const M3<D> X = ld(u, K, s);
     const M3<D> P = p.template load_mat<D>(K, s);
     if (fresh) {
       m = X;
@@ -474,7 +491,7 @@ struct DWalk {
   MAG_D void bwd(int K) {
     shift_rt(g, y, K, -1);
     const int s = lin(g, y);
-    const M3<D> Uy = u.template load<D>(K, s);
+    const M3<D> Uy = ld(u, K, s);
     const M3<D> P = p.template load_mat<D>(K, s);
     if (fresh) {
       m = dag(Uy);
@@ -613,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) MAG_MULTI_OCC void k_multi_xi(MultiArgs a) {
     coarse_site(a, sc, x0);
     with_mu(tl.row, [&](auto mc) {
       constexpr int MU = decltype(mc)::value;
-      acc = retrace_mul_nd(coarse_Gb<MU>(a, x0), a.C.template load<D>(MU, sc)) * xi_at(a, MU, sc);
+      acc = retrace_mul_nd(coarse_Gb<MU>(a, x0), ld(a.C, MU, sc)) * xi_at(a, MU, sc);
     });
   }
   const double t = block_sum(acc);
@@ -643,7 +660,7 @@ MAG_D void heatbath_link(const MultiArgs& a, const int* x0, int sc) {
     case 1: m(0, 0) = {a0, a3}; m(0, 2) = {a2, a1}; m(2, 0) = {-a2, a1}; m(2, 2) = {a0, -a3}; break;
     default: m(0, 0) = {a0, a3}; m(0, 1) = {a2, a1}; m(1, 0) = {-a2, a1}; m(1, 1) = {a0, -a3}; break;
   }
-  const M3<D> var = a.C.template load<D>(MU, sc);
+  const M3<D> var = ld(a.C, MU, sc);
   const M3<D> nv = mul(var, m);
   const double diff = a.coeff * retrace(sub(mul_nd(nv, env), mul_nd(var, env)));
   if (rng_u01(w2.z, w2.w) < exp(diff)) a.C.store(MU, sc, nv);
