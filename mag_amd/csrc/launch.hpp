@@ -68,7 +68,7 @@ int launch(mag_ctx c, const char* timer, void (*ks)(A), void (*kb)(const A*), di
 // Device-to-device copy of `bytes` (a multiple of 16) through a kernel.
 inline int copy_async(mag_ctx c, void* dst, const void* src, size_t bytes) {
   CopyArgs a{(double2*)dst, (const double2*)src, bytes / 16};
-  const unsigned nb = (unsigned)std::min<size_t>((a.n + BLOCK - 1) / BLOCK, 4096);
+  const unsigned nb = (unsigned)std::min<size_t>((a.n + BLOCK - 1) / BLOCK, (size_t)1 << 30);
   return launch(c, "copy", k_copy<0>, k_copy_b<0>, dim3(nb ? nb : 1), a);
 }
 

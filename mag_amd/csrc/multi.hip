@@ -794,18 +794,23 @@ int xi_sum(const Multi& m, void* U, void* C, const void* xi, int slot) {
   return final_reduce(m.c, m.c->d_part, 4 * nb, 1, 0, slot);
 }
 
-int coarse_update(const Multi& m, void* C, void* Cm, double eps) {
-  return Ops<double, 18>::link_update(m.c, m.gc, C, C, Cm, eps, m.mag);
+// Link updates C <- e^{i eps Cm} Cin and U <- e^{i eps P} Uin (Uin = U unless
+// given: an update out of a copy source replaces the copy).
+int coarse_update(const Multi& m, void* C, void* Cm, double eps, const void* Cin = nullptr) {
+  return Ops<double, 18>::link_update(m.c, m.gc, Cin ? const_cast<void*>(Cin) : C, C, Cm, eps, m.mag);
 }
-int fine_update(const Multi& m, void* U, void* P, double eps) {
-  return Ops<double, 18>::link_update(m.c, m.gf, U, U, P, eps, m.mag);
+int fine_update(const Multi& m, void* U, void* P, double eps, const void* Uin = nullptr) {
+  return Ops<double, 18>::link_update(m.c, m.gf, Uin ? const_cast<void*>(Uin) : U, U, P, eps, m.mag);
 }
 
-// force_gradient_integrator_zeta_T (field-multi-md.h:441-475)
-int zeta_T(const Multi& m, void* U, void* P, void* C, void* Cm, double dt) {
+// force_gradient_integrator_zeta_T (field-multi-md.h:441-475). With U0 / C0
+// the fine / coarse fields start as copies of U0 / C0 (the first update of
+// each reads them).
+int zeta_T(const Multi& m, void* U, void* P, void* C, void* Cm, double dt, const void* U0 = nullptr,
+           const void* C0 = nullptr) {
   int r;
-  if ((r = fine_update(m, U, P, dt / 6.))) return r;
-  if ((r = cforce(m, Cm, U, P)) || (r = coarse_update(m, C, Cm, 0.5 * dt))) return r;
+  if ((r = fine_update(m, U, P, dt / 6., U0))) return r;
+  if ((r = cforce(m, Cm, U, P)) || (r = coarse_update(m, C, Cm, 0.5 * dt, C0))) return r;
   if ((r = fine_update(m, U, P, 2. * dt / 3.))) return r;
   if ((r = cforce(m, Cm, U, P)) || (r = coarse_update(m, C, Cm, 0.5 * dt))) return r;
   return fine_update(m, U, P, dt / 6.);
@@ -823,7 +828,6 @@ int zeta_H(const Multi& m, void* U, void* P, void* C, const void* xi, const mag_
   int r;
   if ((r = m.wf->need_md()) || (r = m.wc->need_md())) return r;
   void *Ff = m.wf->F, *FgA = m.wf->Uaux, *CgA = m.wc->Uaux, *Cm = m.wc->F;
-  const size_t fb = (size_t)m.gf.V * 4 * 18 * 8, cb = (size_t)m.gc.V * 4 * 18 * 8;
   const double alpha = (3. - std::sqrt(3.)) * dt / 6.;
   const double beta = dt / std::sqrt(3.);
   const double gamma = (2. - std::sqrt(3.)) * dt * dt / 12.;
@@ -831,8 +835,7 @@ int zeta_H(const Multi& m, void* U, void* P, void* C, const void* xi, const mag_
   for (int i = 0; i < steps; ++i) {
     for (int h = 0; h < 2; ++h) {
       if ((r = fforce(m, Ff, U, C, xi, act))) return r;
-      if ((r = copy_dev(m.c, FgA, U, fb)) || (r = copy_dev(m.c, CgA, C, cb))) return r;
-      if ((r = zeta_T(m, FgA, Ff, CgA, Cm, gamma))) return r;
+      if ((r = zeta_T(m, FgA, Ff, CgA, Cm, gamma, U, C))) return r;  // FgA, CgA from U, C
       if ((r = fforce(m, Ff, FgA, CgA, xi, act))) return r;
       if ((r = mom_update(m.c, m.gf, MAG_F64, P, Ff, 0.5 * dt))) return r;
       if (h == 0 && (r = zeta_T(m, U, P, C, Cm, beta))) return r;
@@ -849,7 +852,6 @@ int fg_md(const Multi& m, void* U, void* P, void* C, const void* xi, const mag_a
   int r;
   if ((r = m.wf->need_md())) return r;
   void *Ff = m.wf->F, *FgA = m.wf->Uaux;
-  const size_t fb = (size_t)m.gf.V * 4 * 18 * 8;
   const double alpha = (3. - std::sqrt(3.)) * dt / 6.;
   const double beta = dt / std::sqrt(3.);
   const double gamma = (2. - std::sqrt(3.)) * dt * dt / 12.;
@@ -857,8 +859,7 @@ int fg_md(const Multi& m, void* U, void* P, void* C, const void* xi, const mag_a
   for (int i = 0; i < steps; ++i) {
     for (int h = 0; h < 2; ++h) {
       if ((r = fforce(m, Ff, U, C, xi, act))) return r;
-      if ((r = copy_dev(m.c, FgA, U, fb))) return r;
-      if ((r = fine_update(m, FgA, Ff, gamma))) return r;
+      if ((r = fine_update(m, FgA, Ff, gamma, U))) return r;  // FgA = e^{i gamma Ff} U
       if ((r = fforce(m, Ff, FgA, C, xi, act))) return r;
       if ((r = mom_update(m.c, m.gf, MAG_F64, P, Ff, 0.5 * dt))) return r;
       if (h == 0 && (r = fine_update(m, U, P, beta))) return r;
