@@ -401,10 +401,12 @@ MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS,
 
 // Top of the chain: Z_n = -(beta/3) S(U_n)^+ (derivative of the gauge action
 // on the smeared field), then the Lambda step of level n-1.
-template <class TS, int NC>
+// NCN: reals per stored link of U_n (FP64 12-real chains store the top level
+// as 18 reals: the Iwasaki staple sum then rebuilds no rows and spills less).
+template <class TS, int NC, int NCN = NC>
 struct StoutTopArgs {
   Geom g;
-  LinkView<TS, NC> Un;      // smeared links U_n
+  LinkView<TS, NCN> Un;     // smeared links U_n
   LinkView<TS, NC> Ulo;     // U_{n-1}
   AlgView<TS> Qlo;
   MatView<TS> Clo;
@@ -413,8 +415,8 @@ struct StoutTopArgs {
   double beta, c1, rho;
 };
 
-template <int ACT, class TS, int NC, class TC>
-MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
+template <int ACT, class TS, int NC, class TC, int NCN>
+MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC, NCN>& a) {
   int mu, sblk;
   stencil_tile<sizeof(TC) == sizeof(double)>(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
@@ -422,14 +424,18 @@ MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC>& a) {
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
   const TC c0 = TC(1) - TC(8) * c1;
   const Site p = site_of(a.g, s);
-  const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.Un, a.g, p, mu, c0, c1);
+  const M3<TC> S = staple_sum<ACT, TS, NCN, TC>(a.Un, a.g, p, mu, c0, c1);
   const M3<TC> Z = scale(dag(S), TC(-a.beta / 3.));
   stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
 }
-template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top(StoutTopArgs<TS, NC> a) { k_stout_top_body<ACT, TS, NC, TC>(a); }
-template <int ACT, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top_b(const StoutTopArgs<TS, NC>* t) { k_stout_top_body<ACT, TS, NC, TC>(t[blockIdx.z]); }
+template <int ACT, class TS, int NC, class TC, int NCN = NC>
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top(StoutTopArgs<TS, NC, NCN> a) {
+  k_stout_top_body<ACT, TS, NC, TC, NCN>(a);
+}
+template <int ACT, class TS, int NC, class TC, int NCN = NC>
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top_b(const StoutTopArgs<TS, NC, NCN>* t) {
+  k_stout_top_body<ACT, TS, NC, TC, NCN>(t[blockIdx.z]);
+}
 
 // Backward level k (eq. 75): Z_k = W_k + sum over nu of the six
 // Lambda-inserted staple terms. With Y = -i rho U^+ Lambda every Y sits next

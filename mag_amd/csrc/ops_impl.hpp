@@ -67,31 +67,42 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   }
   const int n = act.n_stout;
   if ((r = ws->need_stout(n))) return r;
+  // FP64 12-real chains keep the top level U_n as 18 reals (written by the
+  // last forward level, read only by the top kernel's staple sum).
+  constexpr int NCN = (std::is_same<TS, double>::value && NC == 12) ? 18 : NC;
+  if (NCN != NC && (r = ws->need_top18())) return r;
   auto Ulev = [&](int k) { return k == 0 ? U : ws->Us[k - 1]; };
   for (int k = 0; k < n; ++k) {
-    StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g), mview<TS>(ws->Cs[k], g),
-                                    lview<TS, NC>(ws->Us[k], g), act.rho, 1};
-    if ((r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>, grid_stencil(g),
-                    fw)))
-      return r;
+    if (NCN != NC && k == n - 1) {
+      StoutFwdArgs<TS, NC, TS, NCN> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g),
+                                       mview<TS>(ws->Cs[k], g), lview<TS, NCN>(ws->Utop, g), act.rho, 1};
+      r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NCN, TS>, k_stout_fwd_b<TS, NC, TS, NCN, TS>,
+                 grid_stencil(g), fw);
+    } else {
+      StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g),
+                                      mview<TS>(ws->Cs[k], g), lview<TS, NC>(ws->Us[k], g), act.rho, 1};
+      r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>,
+                 grid_stencil(g), fw);
+    }
+    if (r) return r;
   }
   int cur = 0;
   {
-    StoutTopArgs<TS, NC> tp{g,
-                            lview<TS, NC>(Ulev(n), g),
-                            lview<TS, NC>(Ulev(n - 1), g),
-                            aview<TS>(ws->Qs[n - 1], g),
-                            mview<TS>(ws->Cs[n - 1], g),
-                            aview<TS>(ws->L[cur], g),
-                            mview<TS>(ws->W, g),
-                            act.beta,
-                            act.c1,
-                            act.rho};
+    StoutTopArgs<TS, NC, NCN> tp{g,
+                                 lview<TS, NCN>(NCN != NC ? ws->Utop : Ulev(n), g),
+                                 lview<TS, NC>(Ulev(n - 1), g),
+                                 aview<TS>(ws->Qs[n - 1], g),
+                                 mview<TS>(ws->Cs[n - 1], g),
+                                 aview<TS>(ws->L[cur], g),
+                                 mview<TS>(ws->W, g),
+                                 act.beta,
+                                 act.c1,
+                                 act.rho};
     if (iw)
-      r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS>, k_stout_top_b<ACT_IWASAKI, TS, NC, TS>,
-                 grid_stencil(g), tp);
+      r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS, NCN>,
+                 k_stout_top_b<ACT_IWASAKI, TS, NC, TS, NCN>, grid_stencil(g), tp);
     else
-      r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS>, k_stout_top_b<ACT_WILSON, TS, NC, TS>,
+      r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS, NCN>, k_stout_top_b<ACT_WILSON, TS, NC, TS, NCN>,
                  grid_stencil(g), tp);
     if (r) return r;
   }

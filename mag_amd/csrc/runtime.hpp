@@ -245,6 +245,7 @@ struct Workspace {
   void* L[2] = {nullptr, nullptr};  // stout Lambda (alg8), ping-pong between levels
   void* W = nullptr;
   void* Uh[2] = {nullptr, nullptr};  // FP64 smeared fields for the Hamiltonian
+  void* Utop = nullptr;  // FP64 12-real chains: the force's top smeared level, 18 reals
   void* Lam = nullptr;  // alg8: Lambda of the coarse/fine HMC (coarse shape)
   void* Eq = nullptr;   // mat18: e^{iQ} of the coarse/fine HMC (coarse shape)
   void* Cnt = nullptr;  // V doubles: per-thread hit counts of the local Metropolis
@@ -284,6 +285,7 @@ struct Workspace {
     return get(&W, mat_bytes());
   }
   int need_Z() { return get(&Z, mat_bytes()); }
+  int need_top18() { return get(&Utop, mat64_bytes()); }
   int need_cnt() { return get(&Cnt, (size_t)g.V * sizeof(double)); }
   int need_multi() {
     int r = get(&Lam, alg_bytes());
