@@ -280,7 +280,9 @@ struct UpdateArgs {
   int mag;
 };
 
-template <class TS, int NC, class TC>
+// R12: read an 18-real link as its rows 0 and 1 and rebuild row 2 (for
+// callers whose links are unitary to rounding; two thirds of the bytes).
+template <class TS, int NC, class TC, bool R12 = false>
 MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
   const int s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
@@ -288,7 +290,13 @@ MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
   const Site p = site_of(a.g, s);
   const int cls = link_class_rt(p, mu, a.mag);
   const TC e = TC(a.eps);
-  M3<TC> U = a.Uin.template load_nt<TC>(mu, s);
+  M3<TC> U;
+  if constexpr (R12) {
+    U = a.Uin.template load_r2_nt<TC>(mu, s);
+    reconstruct_row2(U);
+  } else {
+    U = a.Uin.template load_nt<TC>(mu, s);
+  }
   TC h[8];
   if (cls == 0 || cls == 1 || cls == 10) {
     a.P.load_nt(mu, s, h);
@@ -306,10 +314,12 @@ MAG_D void k_link_update_body(const UpdateArgs<TS, NC>& a) {
   }
   a.Uout.store_nt(mu, s, U);
 }
-template <class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) { k_link_update_body<TS, NC, TC>(a); }
-template <class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) void k_link_update_b(const UpdateArgs<TS, NC>* t) { k_link_update_body<TS, NC, TC>(t[blockIdx.z]); }
+template <class TS, int NC, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) void k_link_update(UpdateArgs<TS, NC> a) { k_link_update_body<TS, NC, TC, R12>(a); }
+template <class TS, int NC, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) void k_link_update_b(const UpdateArgs<TS, NC>* t) {
+  k_link_update_body<TS, NC, TC, R12>(t[blockIdx.z]);
+}
 
 // P += eps m F (field-hmc.h:427-440), for the unfused path.
 template <class TS>

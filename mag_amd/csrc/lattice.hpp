@@ -216,10 +216,10 @@ struct LinkView {
   template <class TC>
   MAG_D M3<TC> load(int mu, int s) const { return get<TC, false>(mu, s); }
   // rows 0 and 1 only (row 2 zero), for a factor read only there
-  template <class TC>
+  template <class TC, bool NT = false>
   MAG_D M3<TC> load_r2(int mu, int s) const {
     TS r[12];
-    PK::template get<12, false>(p, V, mu, s, r);
+    PK::template get<12, NT>(p, V, mu, s, r);
     M3<TC> m;
 #pragma unroll
     for (int k = 0; k < 6; ++k) m.e[k] = {TC(r[2 * k]), TC(r[2 * k + 1])};
@@ -227,6 +227,8 @@ struct LinkView {
     for (int k = 6; k < 9; ++k) m.e[k] = {TC(0), TC(0)};
     return m;
   }
+  template <class TC>
+  MAG_D M3<TC> load_r2_nt(int mu, int s) const { return load_r2<TC, true>(mu, s); }
   template <class TC>
   MAG_D void store(int mu, int s, const M3<TC>& m) const { put<TC, false>(mu, s, m); }
   // Non-temporal forms for data a kernel touches once, at its own link

@@ -795,12 +795,22 @@ int xi_sum(const Multi& m, void* U, void* C, const void* xi, int slot) {
 }
 
 // Link updates C <- e^{i eps Cm} Cin and U <- e^{i eps P} Uin (Uin = U unless
-// given: an update out of a copy source replaces the copy).
+// given: an update out of a copy source replaces the copy). The links are
+// read through ld()'s rule (rows 0 and 1 with MAG_MD_READ12).
+int md_update(const Multi& m, const Geom& g, const void* Uin, void* Uout, void* P, double eps) {
+  if (m.c->rec) return fail(m.c, MAG_ERR_UNSUPPORTED, "the coarse/fine HMC kernels are not recordable");
+  const UpdateArgs<double, 18> a{g, LV{(double*)Uin, g.V}, LV{(double*)Uout, g.V}, AV{(double*)P, g.V}, eps, m.mag};
+  KTimer t(m.c, "link_update");
+  hipLaunchKernelGGL((k_link_update<double, 18, double, MAG_MD_READ12 != 0>), grid_links(g), dim3(BLOCK), 0,
+                     m.c->stream, a);
+  CHECK_LAUNCH(m.c);
+  return MAG_OK;
+}
 int coarse_update(const Multi& m, void* C, void* Cm, double eps, const void* Cin = nullptr) {
-  return Ops<double, 18>::link_update(m.c, m.gc, Cin ? const_cast<void*>(Cin) : C, C, Cm, eps, m.mag);
+  return md_update(m, m.gc, Cin ? Cin : C, C, Cm, eps);
 }
 int fine_update(const Multi& m, void* U, void* P, double eps, const void* Uin = nullptr) {
-  return Ops<double, 18>::link_update(m.c, m.gf, Uin ? const_cast<void*>(Uin) : U, U, P, eps, m.mag);
+  return md_update(m, m.gf, Uin ? Uin : U, U, P, eps);
 }
 
 // force_gradient_integrator_zeta_T (field-multi-md.h:441-475). With U0 / C0
