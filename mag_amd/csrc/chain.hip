@@ -166,6 +166,7 @@ size_t chain_footprint(const int dims[4], const mag_action& act, const mag_hmc_p
   if (act.n_stout > 0) {
     b += (size_t)act.n_stout * (link + alg + mat) + 2 * alg + mat;  // levels; Lambda[2], W
     b += (act.n_stout > 1 ? 2 : 1) * mat64;                    // FP64 smearing for H
+    if (!f32 && p.recon12) b += mat64;                         // FP64 12-real: top level as 18 reals
     if (act.mag) b += mat;
   }
   b += ((V + 255) / 256 * 4 + 4096) * sizeof(double) + 64 * sizeof(double);

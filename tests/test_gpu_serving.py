@@ -146,3 +146,27 @@ def test_poisson_stream_small(ctx):
     for arr, t, res in out.values():
         assert all(np.isfinite(x.h_new) for x in res)
     assert st["p99_traj_latency_ms"] >= st["p50_traj_latency_ms"] > 0
+
+
+@pytest.mark.parametrize("prec,recon12,n_stout", [(M.F32, False, 0), (M.F32, False, 3), (M.F64, False, 1),
+                                                  (M.F64, False, 3), (M.F64, True, 0), (M.F64, True, 3)])
+def test_footprint_covers_chain_memory(prec, recon12, n_stout):
+    """Admission charges mag_server_footprint per chain: it must cover what a
+    chain of that class allocates once every workspace is in use (one
+    trajectory run), and not overstate it by much."""
+    L = (8, 8, 8, 8)
+    req = S.request(L, M.IWASAKI, 2.6, -0.331, n_stout=n_stout, ntraj=1, n_md=2, seed=5, precision=prec,
+                    recon12=recon12)
+    fp = S.footprint(req)
+    c = M.Context(0)
+    try:
+        m0 = c.memory()
+        arg = M.Arg_chmc(trajectory_length=2, dt=0.5, beta=2.6, gauge=M.Gauge(1, -0.331), n_stout=n_stout, rho=0.1)
+        ch = M.Chain(c, L, arg, seed=5, precision=prec, recon12=recon12)
+        ch.haar(5)
+        ch.run(1)
+        used = c.memory() - m0
+        assert used <= fp, (used, fp)
+        assert fp <= 1.2 * used, (used, fp)
+    finally:
+        c.close()
