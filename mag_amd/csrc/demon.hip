@@ -267,6 +267,11 @@ struct DemonClassArgs {
 // 64 lanes per member (every lane's gather its own cache lines), 155 ms with
 // 16, 142 ms with 8, 193 ms with 4 (too few wavefronts per class).
 constexpr int DEMON_LPL = 8;
+// MAG_DEMON_READ12 = 1: the path links are read as rows 0 and 1 with row 2
+// rebuilt (the gathers bound the class kernel; the links stay in SU(3)).
+#ifndef MAG_DEMON_READ12
+#define MAG_DEMON_READ12 1
+#endif
 template <int MU, int LPL>
 __global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
   constexpr int MPW = 64 / LPL;
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
           if (i == k) y[i] = (y[i] == 0) ? A.g.L[i] - 1 : y[i] - 1;
       }
       const int sy = y[0] + A.g.S[1] * y[1] + A.g.S[2] * y[2] + A.g.S[3] * y[3];
-      const M3<double> l = A.U.template load<double>(k, sy);
+      const M3<double> l = A.U.template load_su3<double, MAG_DEMON_READ12 != 0>(k, sy);
       if (fresh) m = (n > 0) ? l : dag(l);
       else m = (n > 0) ? mul(m, l) : mul_nd(m, l);
       fresh = false;

@@ -229,6 +229,18 @@ struct LinkView {
   }
   template <class TC>
   MAG_D M3<TC> load_r2_nt(int mu, int s) const { return load_r2<TC, true>(mu, s); }
+  // A link known to be in SU(3): with R12 an 18-real link is read as rows 0
+  // and 1 (two thirds of its groups) and row 2 rebuilt as conj(r0 x r1).
+  template <class TC, bool R12>
+  MAG_D M3<TC> load_su3(int mu, int s) const {
+    if constexpr (R12 && NC == 18) {
+      M3<TC> m = load_r2<TC>(mu, s);
+      reconstruct_row2(m);
+      return m;
+    } else {
+      return load<TC>(mu, s);
+    }
+  }
   template <class TC>
   MAG_D void store(int mu, int s, const M3<TC>& m) const { put<TC, false>(mu, s, m); }
   // Non-temporal forms for data a kernel touches once, at its own link

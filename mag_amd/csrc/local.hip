@@ -58,6 +58,11 @@ MAG_D void shift_rt(const Geom& g, int* y, int k, int n) {
 }
 MAG_D int lin(const Geom& g, const int* y) { return y[0] + g.S[1] * y[1] + g.S[2] * y[2] + g.S[3] * y[3]; }
 
+// MAG_LOCAL_READ12 = 1: staple links read as rows 0 and 1, row 2 rebuilt.
+#ifndef MAG_LOCAL_READ12
+#define MAG_LOCAL_READ12 1
+#endif
+
 // Plaquette staple sum sigma of link (x, mu): Re Tr(U_mu(x) sigma) is the sum
 // of the six plaquettes through the link (get_staple_dagger, field-hmc.h:203-242).
 MAG_D M3<D> staple(const LocalArgs& A, const int* x, int mu) {
@@ -67,18 +72,18 @@ MAG_D M3<D> staple(const LocalArgs& A, const int* x, int mu) {
     if (nu == mu) continue;
     int y[4] = {x[0], x[1], x[2], x[3]};
     shift_rt(A.g, y, mu, 1);
-    const M3<D> a = A.U.template load<D>(nu, lin(A.g, y));  // U_nu(x+mu)
+    const M3<D> a = A.U.template load_su3<D, MAG_LOCAL_READ12 != 0>(nu, lin(A.g, y));  // U_nu(x+mu)
     int z[4] = {x[0], x[1], x[2], x[3]};
     shift_rt(A.g, z, nu, 1);
-    const M3<D> b = A.U.template load<D>(mu, lin(A.g, z));  // U_mu(x+nu)
-    const M3<D> c = A.U.template load<D>(nu, lin(A.g, x));  // U_nu(x)
+    const M3<D> b = A.U.template load_su3<D, MAG_LOCAL_READ12 != 0>(mu, lin(A.g, z));  // U_mu(x+nu)
+    const M3<D> c = A.U.template load_su3<D, MAG_LOCAL_READ12 != 0>(nu, lin(A.g, x));  // U_nu(x)
     mul_nd_acc(s, mul_nd(a, b), c);                          // U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
     shift_rt(A.g, y, nu, -1);                                // x+mu-nu
     int w[4] = {x[0], x[1], x[2], x[3]};
     shift_rt(A.g, w, nu, -1);                                // x-nu
-    const M3<D> d = A.U.template load<D>(nu, lin(A.g, y));  // U_nu(x+mu-nu)
-    const M3<D> e = A.U.template load<D>(mu, lin(A.g, w));  // U_mu(x-nu)
-    const M3<D> f = A.U.template load<D>(nu, lin(A.g, w));  // U_nu(x-nu)
+    const M3<D> d = A.U.template load_su3<D, MAG_LOCAL_READ12 != 0>(nu, lin(A.g, y));  // U_nu(x+mu-nu)
+    const M3<D> e = A.U.template load_su3<D, MAG_LOCAL_READ12 != 0>(mu, lin(A.g, w));  // U_mu(x-nu)
+    const M3<D> f = A.U.template load_su3<D, MAG_LOCAL_READ12 != 0>(nu, lin(A.g, w));  // U_nu(x-nu)
     mul_acc(s, mul_dd(d, e), f);                             // U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
   }
   return s;

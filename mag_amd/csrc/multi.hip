@@ -87,15 +87,7 @@ using IC = std::integral_constant<int, K>;
 #ifndef MAG_MD_READ12
 #define MAG_MD_READ12 1
 #endif
-MAG_D M3<D> ld(const LV& v, int mu, int s) {
-  if constexpr (MAG_MD_READ12) {
-    M3<D> m = v.template load_r2<D>(mu, s);
-    reconstruct_row2(m);
-    return m;
-  } else {
-    return v.template load<D>(mu, s);
-  }
-}
+MAG_D M3<D> ld(const LV& v, int mu, int s) { return v.template load_su3<D, MAG_MD_READ12 != 0>(mu, s); }
 
 // Shift of a fine coordinate along the compile-time direction K, wrapped
 // into [0, L) (|n| <= 2 < L).
