@@ -147,6 +147,30 @@ dim3 grid_of(const Geom& g) { return dim3((g.V + BLOCK - 1) / BLOCK, 4, 1); }
 
 }  // namespace
 
+namespace mag {
+// Sum of Re Tr of the mag x mag loops at the coarse sites into scalar slot
+// `slot` (no host read); *norm turns it into the constrained plaquette.
+int constrained_plaq_sum(mag_ctx c, const Geom& g, int prec, int kind, void* U, int mag, int slot, double* norm) {
+  int dc[4];
+  for (int i = 0; i < 4; ++i) dc[i] = g.L[i] / mag;
+  const Geom gc = make_geom(dc);
+  const int nb = (gc.V + BLOCK - 1) / BLOCK;
+  int r;
+  if ((r = ensure_part(c, (size_t)nb))) return r;
+  r = dispatch_link(c, prec, kind, [&](auto t) -> int {
+    using TS = typename decltype(t)::T;
+    constexpr int N = decltype(t)::N;
+    hipLaunchKernelGGL((k_constrained_plaq<TS, N>), dim3(nb), dim3(BLOCK), 0, c->stream,
+                       LinkView<TS, N>{(TS*)U, g.V}, g, gc, mag, c->d_part);
+    CHECK_LAUNCH(c);
+    return MAG_OK;
+  });
+  if (r) return r;
+  *norm = 1. / (3. * 6. * (double)gc.V);
+  return final_reduce(c, c->d_part, nb, 1, 0, slot);
+}
+}  // namespace mag
+
 extern "C" {
 
 int mag_coarsen_links(mag_field fine, int mag, mag_field coarse) {
@@ -182,25 +206,12 @@ int mag_update_constrain(mag_field fine, mag_field coarse, int mag) {
 int mag_constrained_plaquette(mag_field U, int mag, double* out) {
   if (!linkf(U) || mag < 1) return MAG_ERR_ARG;
   mag_ctx c = U->ctx;
-  int dc[4];
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 4; ++i)
     if (U->dims[i] % mag) return fail(c, MAG_ERR_GEOMETRY, "extents must be multiples of mag");
-    dc[i] = U->dims[i] / mag;
-  }
-  const Geom gc = make_geom(dc);
-  const int nb = (gc.V + BLOCK - 1) / BLOCK;
-  int r;
-  if ((r = ensure_part(c, (size_t)nb))) return r;
-  r = dispatch_link(c, U->prec, U->kind, [&](auto t) -> int {
-    using TS = typename decltype(t)::T;
-    constexpr int N = decltype(t)::N;
-    hipLaunchKernelGGL((k_constrained_plaq<TS, N>), dim3(nb), dim3(BLOCK), 0, c->stream,
-                       LinkView<TS, N>{(TS*)U->p, U->g.V}, U->g, gc, mag, c->d_part);
-    CHECK_LAUNCH(c);
-    return MAG_OK;
-  });
-  if (r || (r = final_reduce(c, c->d_part, nb, 1, 0, 0)) || (r = fetch_scalars(c, 1))) return r;
-  if (out) *out = c->h_scalar[0] / (3. * 6. * (double)gc.V);
+  double norm = 0.;
+  int r = mag::constrained_plaq_sum(c, U->g, U->prec, U->kind, U->p, mag, 0, &norm);
+  if (r || (r = fetch_scalars(c, 1))) return r;
+  if (out) *out = c->h_scalar[0] * norm;
   return MAG_OK;
 }
 

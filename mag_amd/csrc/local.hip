@@ -23,6 +23,7 @@
 // of link (x, mu) takes draws 3h, 3h+1, 3h+2 (a, b, c, d, select, die in
 // [-1, 1), as CPS's LRG with SetInterval(1, -1)); a pair draws from its
 // first link. Hit counts go to per-thread slots (no atomics), summed at the end.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -252,6 +253,15 @@ int launch_local(mag_ctx c, void (*k)(LocalArgs), dim3 grid, dim3 block, const L
 
 }  // namespace
 
+// dst = w * (scalar slot 0), on the device
+static int record_scalar(mag_ctx c, double* dst, double w) {
+  LinCombArgs a{c->d_scalar, dst, 1, {}};
+  a.w[0][0] = w;
+  hipLaunchKernelGGL(k_lincomb<0>, dim3(1), dim3(64), 0, c->stream, a);
+  CHECK_LAUNCH(c);
+  return MAG_OK;
+}
+
 extern "C" {
 
 int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nhits, double small, uint64_t seed,
@@ -280,6 +290,8 @@ int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nh
   if ((r = ws->need_cnt())) return r;
   double* cnt = (double*)ws->Cnt;
   if ((r = fill_dev(c, cnt, 0., 4 * nthr))) return r;
+  if ((plaq || cons_plaq) && (r = ws->need_rec(2 * (size_t)std::max(num_iter, 1)))) return r;
+  double* rec0 = (double*)ws->Rec;
   LocalArgs A{};
   A.g = g;
   A.U = LV{(double*)U->p, g.V};
@@ -307,15 +319,29 @@ int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nh
           for (A.b = 0; A.b < 2; ++A.b)
             if ((r = launch_local(c, ks[mu], grid, dim3(BLOCK), A))) return r;
     }
-    double dev;
-    if ((r = mag_reunitarize(U, &dev))) return r;
-    if (plaq && (r = mag_avg_plaquette(U, &plaq[it]))) return r;
-    if (cons_plaq) {
-      if (mag > 0) {
-        if ((r = mag_constrained_plaquette(U, mag, &cons_plaq[it]))) return r;
-      } else {
-        cons_plaq[it] = 0.;
-      }
+    // reunitarize, then record the plaquette and the constrained plaquette in
+    // device memory (read once after the last iteration, not per iteration)
+    if ((r = Ops<double, 18>::reunitarize(c, g, U->p, 0))) return r;
+    double* rec = rec0 + 2 * (size_t)it;
+    if (plaq) {
+      if ((r = Ops<double, 18>::smeared_action_sums(c, g, U->p, 0., 0, false, 0)) ||
+          (r = record_scalar(c, rec, 1. / (18. * (double)g.V))))
+        return r;
+    }
+    if (cons_plaq && mag > 0) {
+      double norm = 0.;
+      if ((r = constrained_plaq_sum(c, g, MAG_F64, MAG_LINK18, U->p, mag, 0, &norm)) ||
+          (r = record_scalar(c, rec + 1, norm)))
+        return r;
+    }
+  }
+  if ((plaq || cons_plaq) && num_iter > 0) {
+    std::vector<double> h((size_t)2 * num_iter);
+    HIPCHK(c, hipMemcpyAsync(h.data(), rec0, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int it = 0; it < num_iter; ++it) {
+      if (plaq) plaq[it] = h[2 * (size_t)it];
+      if (cons_plaq) cons_plaq[it] = mag > 0 ? h[2 * (size_t)it + 1] : 0.;
     }
   }
   if (acc_rej) {

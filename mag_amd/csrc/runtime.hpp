@@ -249,6 +249,8 @@ struct Workspace {
   void* Lam = nullptr;  // alg8: Lambda of the coarse/fine HMC (coarse shape)
   void* Eq = nullptr;   // mat18: e^{iQ} of the coarse/fine HMC (coarse shape)
   void* Cnt = nullptr;  // V doubles: per-thread hit counts of the local Metropolis
+  void* Rec = nullptr;  // per-iteration records of the local Metropolis (grows)
+  size_t rec_cap = 0;
 
   ~Workspace() {
     for (auto& a : allocs) dev_free(c, a.first, a.second);
@@ -287,6 +289,21 @@ struct Workspace {
   int need_Z() { return get(&Z, mat_bytes()); }
   int need_top18() { return get(&Utop, mat64_bytes()); }
   int need_cnt() { return get(&Cnt, (size_t)g.V * sizeof(double)); }
+  int need_rec(size_t n) {
+    if (Rec && rec_cap >= n) return MAG_OK;
+    if (Rec) {
+      for (auto it = allocs.begin(); it != allocs.end(); ++it)
+        if (it->first == Rec) {
+          dev_free(c, it->first, it->second);
+          allocs.erase(it);
+          break;
+        }
+      Rec = nullptr;
+    }
+    int r = get(&Rec, n * sizeof(double));
+    rec_cap = r ? 0 : n;
+    return r;
+  }
   int need_multi() {
     int r = get(&Lam, alg_bytes());
     if (r) return r;
@@ -374,6 +391,7 @@ int batch_dexpiq(mag_ctx c, const double* dQ, const double* dDQ, double* dout, l
 double host_uniform(uint64_t seed, uint32_t stream, uint64_t index);
 // noalloc.hip: resolve the allocation-free scratch array of `device` (current)
 int noalloc_setup(int device, std::string* err);
+int constrained_plaq_sum(mag_ctx c, const Geom& g, int prec, int kind, void* U, int mag, int slot, double* norm);
 
 inline double action_from_sums(const mag_action& act, double sp, double sr) {
   // field-hmc.h:632-640
