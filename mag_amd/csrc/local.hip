@@ -290,8 +290,11 @@ int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nh
   if ((r = ws->need_cnt())) return r;
   double* cnt = (double*)ws->Cnt;
   if ((r = fill_dev(c, cnt, 0., 4 * nthr))) return r;
-  if ((plaq || cons_plaq) && (r = ws->need_rec(2 * (size_t)std::max(num_iter, 1)))) return r;
-  double* rec0 = (double*)ws->Rec;
+  double* rec0 = nullptr;  // per iteration: plaquette, constrained plaquette
+  if (plaq || cons_plaq) {
+    if ((r = ws->need_rec(2 * (size_t)std::max(num_iter, 1)))) return r;
+    rec0 = (double*)ws->Rec;
+  }
   LocalArgs A{};
   A.g = g;
   A.U = LV{(double*)U->p, g.V};
@@ -322,16 +325,15 @@ int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nh
     // reunitarize, then record the plaquette and the constrained plaquette in
     // device memory (read once after the last iteration, not per iteration)
     if ((r = Ops<double, 18>::reunitarize(c, g, U->p, 0))) return r;
-    double* rec = rec0 + 2 * (size_t)it;
     if (plaq) {
       if ((r = Ops<double, 18>::smeared_action_sums(c, g, U->p, 0., 0, false, 0)) ||
-          (r = record_scalar(c, rec, 1. / (18. * (double)g.V))))
+          (r = record_scalar(c, rec0 + 2 * (size_t)it, 1. / (18. * (double)g.V))))
         return r;
     }
     if (cons_plaq && mag > 0) {
       double norm = 0.;
       if ((r = constrained_plaq_sum(c, g, MAG_F64, MAG_LINK18, U->p, mag, 0, &norm)) ||
-          (r = record_scalar(c, rec + 1, norm)))
+          (r = record_scalar(c, rec0 + 2 * (size_t)it + 1, norm)))
         return r;
     }
   }
