@@ -97,6 +97,9 @@ def parse():
                    help="time the coupled coarse/fine md:: HMC (FP64) instead of the headline workload")
     p.add_argument("--no-reversibility", dest="reversibility", action="store_false",
                    help="skip the FP64 reversibility round trip after the timed run")
+    p.add_argument("--shard", action="store_true",
+                   help="time one GPU's share of the 64^4 DBW2 + 3-stout workload split over 8 GPUs along x3 "
+                        "(a 64^3 x 8 block, periodic wrap in place of halos) and model its halo exchange")
     p.add_argument("--fp64-kernel-times", action="store_true",
                    help="also time the kernels of the FP64 chains (HIP events around each launch)")
     return p.parse_args()
@@ -441,8 +444,65 @@ def local_main(a):
     print(json.dumps(out))
 
 
+def shard_main(a):
+    """One GPU's share of the 64^4 DBW2 + 3-stout, n = 16 workload decomposed over 8 GPUs along x3:
+    the unchanged FP32-MD trajectory on a 64^3 x 8 block whose periodic wrap stands in for the halo.
+    Hot start with forced accepts (a throughput measurement, like the 64^4 leg). Prints one JSON line
+    with the measured force time per evaluation and a model of the halo traffic of the split: per
+    force, each forward level needs U_k one layer deep, the top level U_3 two layers deep (rectangles)
+    and each backward level Lambda_k one layer deep, from each of the two neighbours along x3, over one
+    xGMI link each (153 GB/s)."""
+    import mag_amd as M
+    ctx = M.Context(0)
+    dims = (64, 64, 64, 8)
+    n = 16
+    kind, beta, c1 = ACTIONS["dbw2"]
+    arg = M.Arg_chmc(trajectory_length=n, dt=1.0 / n, beta=beta, gauge=M.Gauge(kind, c1), n_stout=3, rho=0.1,
+                     num_forced_accept_step=1 << 30)
+    ch = M.Chain(ctx, dims, arg, seed=a.seed, precision=M.F32)
+    ch.haar(a.seed)
+    warm = max(a.warmup, 1)
+    ch.run(warm)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    ch.run(a.steps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.set_timing(False)
+    kms = kernel_ms(ctx, a.steps)
+    force_ms = sum(kms.get(f, 0.) for f in ("stout_fwd", "stout_top", "stout_bwd", "stout_bwd_force")) / (4 * n)
+    face = dims[0] * dims[1] * dims[2]  # sites per x3 layer
+    link12, alg8 = 4 * 12 * 4, 4 * 8 * 4  # bytes per site: 4 links of 12 FP32 reals; 4 algebra elements
+    per_side = 3 * face * link12 + 2 * face * link12 + 3 * face * alg8  # U_0..U_2, U_3 x 2 layers, Lambda_2..0
+    xgmi = 153e9
+    out = {"metric": "traj/s of one GPU's share of 64^4 DBW2+3stout over 8 GPUs (64^3 x 8 block)", "value":
+           round(a.steps / dt, 4), "unit": "traj/s", "n_gpus": 1, "steps": a.steps, "warmup": warm,
+           "ms_per_step": round(1e3 * dt / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (Haar-random links, seed %d, forced accepts)" % a.seed,
+           "config": {"workload": "64^3 x 8 DBW2(c1=-1.4069, beta=1.04) + 3-stout(rho=0.1), FG tau=1 n=16, FP32 "
+                                  "12-real links, periodic wrap in place of halos", "lattice": list(dims),
+                      "parallelism": "single GPU (one share of an 8-way split along x3)"},
+           "force_ms": round(force_ms, 3), "kernel_ms_per_traj": kms,
+           "halo_model": {"bytes_per_force_per_neighbour": per_side,
+                          "xgmi_gbs_per_link": xgmi / 1e9,
+                          "exchange_ms_per_force": round(1e3 * per_side / xgmi, 3),
+                          "exchange_over_force": round(1e3 * per_side / xgmi / force_ms, 3),
+                          "note": "two neighbours along x3 over two links at once; faces of 64^3 sites; "
+                                  "interior layers 2..5 of 8 do not need the halo and can overlap it"}}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
+    if a.shard:
+        if a.gpus != 1:
+            print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)",
+                  file=sys.stderr)
+            sys.exit(2)
+        shard_main(a)
+        return
     if a.local:
         if a.gpus != 1:
             print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (single-GPU build)",
