@@ -31,13 +31,15 @@ constexpr int BLOCK = 256;
 // FP32 instances are held to 3 waves so that none falls to 2 (which cost
 // the Iwasaki top kernel 20 %). FP64 stout kernels are held to 2 waves:
 // unbounded they take 256 VGPRs plus 4-20 AGPRs and run 1 wave; at 2 waves
-// they spill 20-88 bytes per lane and the backward and top kernels run 16 %
-// and 9 % faster. The FP64 thin-link force of constrained runs (two staple
-// sums for the constraint classes) would spill 536 bytes at 2 waves and keeps 1.
+// they spill 20-116 bytes per lane and the backward and top kernels ran 16 %
+// and 9 % faster. The FP64 thin-link force runs at 1 wave without spills: at
+// 2 it spilled 116 bytes (free links) or 536 (constraint classes) and was 6 %
+// slower on the grouped layout (the top kernel, with the same staple sum,
+// runs equally fast at 1 and 2 waves there).
 template <class TC>
 constexpr int stencil_waves() { return sizeof(TC) == sizeof(float) ? 3 : 2; }
 template <class TC, bool CONS>
-constexpr int gforce_waves() { return sizeof(TC) == sizeof(float) ? 3 : (CONS ? 1 : 2); }
+constexpr int gforce_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
 #define MAG_STENCIL_OCC __attribute__((amdgpu_waves_per_eu(stencil_waves<TC>())))
 #define MAG_GFORCE_OCC __attribute__((amdgpu_waves_per_eu(gforce_waves<TC, CONS>())))
 
@@ -220,8 +222,7 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 // free links, W = U(x+mu) S(x+mu)^+ - S(x)^+ U(x) for constraint classes 1 and
 // 10, and F = 0 for class 100. Stored as F_a = -(beta/3) Im Tr(T_a W).
 // CONS = false (mag = 0) compiles the free-link form alone (one staple sum,
-// no class loop, U(x) loaded after it); its FP64 instance is held to 2 waves
-// per SIMD (88 bytes of spill), 13 % faster than the class-loop kernel at 1.
+// no class loop, U(x) loaded after it).
 template <int ACT, bool CONS, class TS, int NC, class TC>
 MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   int mu, sblk;
