@@ -409,20 +409,19 @@ MAG_D void add_stout_term(const MultiArgs& a, const int* x, const M3<D>& t) {
   a.F.store(MU, s, f);
 }
 
-// Two kernels: the heavy classes 1 and 3 (rows 0-3 class 1, rows 4-7 class 3,
-// direction row & 3) and the light classes 2 and 4 (rows 0-11 class 2, rows
-// 12-23 class 4, (mu, nu) pair row % 12), so the light classes are not held
-// to the heavy ones' register footprint. ctile puts all rows of a coarse
-// block back to back on one XCD, so both classes of a kind read the fine
-// links they share from its L2 (one launch per class took 33 % longer).
-template <bool HEAVY>
+// One launch of 32 rows: rows 0-3 class 1 and 4-7 class 3 (direction row & 3),
+// rows 8-19 class 2 and 20-31 class 4 ((mu, nu) pair (row - 8) % 12). ctile
+// puts the 32 rows of a coarse block back to back on one XCD, so all classes
+// of a coarse site read the fine links they share from its L2 (one launch per
+// class took 1.4x as long, one per pair of classes 1.05x, though the light
+// classes would run 2 waves per SIMD on their own).
 __global__ __launch_bounds__(BLOCK) MAG_FFORCE_OCC void k_multi_fforce(MultiArgs a) {
   const CTile tl = ctile(a);
   const int sc = tl.blk * BLOCK + threadIdx.x;
   if (sc >= a.gc.V) return;
   int x[4];
   coarse_site(a, sc, x);
-  if constexpr (HEAVY) {
+  if (tl.row < 8) {
     with_mu(tl.row & 3, [&](auto mc) {
       constexpr int MU = decltype(mc)::value;
       if (tl.row < 4) {
@@ -433,11 +432,11 @@ __global__ __launch_bounds__(BLOCK) MAG_FFORCE_OCC void k_multi_fforce(MultiArgs
       }
     });
   } else {
-    const int k = tl.row < 12 ? tl.row : tl.row - 12;
-    with_mu_nu(k, [&](auto mc, auto nc) {
+    const int r = tl.row - 8;
+    with_mu_nu(r < 12 ? r : r - 12, [&](auto mc, auto nc) {
       constexpr int MU = decltype(mc)::value, NU = decltype(nc)::value;
       shift<NU>(a.gf, x, 1);
-      if (tl.row < 12) {
+      if (r < 12) {
         add_stout_term<MU>(a, x, fclass2<MU, NU>(a, x, sc));
       } else {
         shift<MU>(a.gf, x, 1);
@@ -748,8 +747,7 @@ int fforce(const Multi& m, void* F, void* U, void* C, const void* xi, const mag_
   a.eq = LV{(double*)m.wc->Eq, m.gc.V};
   a.F = AV{(double*)F, m.gf.V};
   if ((r = mlaunch(m, "multi_prep", k_multi_prep, cgrid(m, 4), a))) return r;
-  if ((r = mlaunch(m, "multi_fforce", k_multi_fforce<true>, cgrid(m, 8), a))) return r;
-  return mlaunch(m, "multi_fforce", k_multi_fforce<false>, cgrid(m, 24), a);
+  return mlaunch(m, "multi_fforce", k_multi_fforce, cgrid(m, 32), a);
 }
 
 // The four directions are the rows of one launch, so ctile puts a coarse
