@@ -86,6 +86,17 @@ MAG_D void stencil_tile(const Geom& g, int& mu, int& sblk) {
 // ======================================================= staples and forces
 enum ActionKind : int { ACT_WILSON = 0, ACT_IWASAKI = 1 };
 
+// Site-index navigation of the staple sum. GNav works on global site
+// indices; TNav (the LDS-tiled top kernel) pairs each with its index in the
+// block's tile of (x0, x1) rows and a wavefront-uniform flag that says
+// whether the displacement stays in that tile.
+struct GNav {
+  const Geom& g;
+  const Site& p;
+  MAG_D int at() const { return p.s; }
+  MAG_D int off(int mu, int d) const { return off_rt(g, p, mu, d); }
+};
+
 // Undaggered staple sum S of link (x, mu): c0 * (6 plaquette staples)
 // + c1 * (18 rectangle staples), the paths of get_staple_dagger
 // (field-hmc.h:226-240) and get_rectangular_dagger (field-hmc.h:137-201),
@@ -99,83 +110,98 @@ enum ActionKind : int { ACT_WILSON = 0, ACT_IWASAKI = 1 };
 // after the nu loop. 62 instead of 85 3x3 products per link. Every partial
 // path is a product of links, so it is formed in two rows (mul_*_r2) and
 // completed once (reconstruct_row2) before it enters a sum.
-template <int ACT, class TS, int NC, class TC>
-MAG_D M3<TC> staple_sum(const LinkView<TS, NC>& u, const Geom& g, const Site& p, int mu, TC c0, TC c1) {
+//
+// The terms of staple_sum in the plane (mu, nu): S gets the plaquette
+// staples and the tall and backward rectangles, R the forward rectangles
+// (multiplied by U_mu(x+mu)^+ once, after the last plane).
+template <int ACT, class TC, class UV, class NAV, class I>
+MAG_D void staple_plane(const UV& u, const NAV& nv, int mu, int nu, const I& xpm, const I& xmm, const I& xp2m, TC c0,
+                        TC c1, M3<TC>& S, M3<TC>& R) {
+  const auto opn = nv.off(nu, 1), omn = nv.off(nu, -1);
+  const auto xpn = nv.at() + opn, xmn = nv.at() + omn;
+  if (ACT == ACT_WILSON) {
+    // U_nu(x) U_mu(x+nu) U_nu(x+mu)^+
+    add_su3_r2(S, mul_nd_r2(mul_r2(u.template load_r2<TC>(nu, nv.at()), u.template load<TC>(mu, xpn)),
+                            u.template load<TC>(nu, xpm)));
+    // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu)
+    add_su3_r2(S, mul_r2(mul_dn_r2(u.template load<TC>(nu, xmn), u.template load<TC>(mu, xmn)),
+                         u.template load<TC>(nu, xpm + omn)));
+  } else {
+    const auto op2n = nv.off(nu, 2), om2n = nv.off(nu, -2);
+    // ---- upper side of the plane: the paths that end in U_nu(x+mu)^+
+    //   G = c0 U_nu(x) U_mu(x+nu)                                (plaquette)
+    //     + c1 U_nu(x) U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+    (tall rectangle)
+    //     + c1 U_mu(x-mu)^+ U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu)  (backward rectangle)
+    // each an SU(3) product formed in two rows; S += G U_nu(x+mu)^+.
+    {
+      const M3<TC> Un_x = u.template load_r2<TC>(nu, nv.at());
+      const M3<TC> Um_xpn = u.template load<TC>(mu, xpn);
+      const M3<TC> P = mul_r2(Un_x, Um_xpn);
+      // forward rectangle R1' = P U_mu(x+nu+mu) U_nu(x+2mu)^+
+      add_su3_r2(R, mul_nd_r2(mul_r2(P, u.template load<TC>(mu, xpm + opn)), u.template load<TC>(nu, xp2m)));
+      M3<TC> G = zero3<TC>();
+      axpy_su3_r2(G, c0, P);
+      {
+        M3<TC> T = mul_r2(Un_x, u.template load<TC>(nu, xpn));
+        T = mul_r2(T, u.template load<TC>(mu, nv.at() + op2n));
+        axpy_su3_r2(G, c1, mul_nd_r2(T, u.template load<TC>(nu, xpm + opn)));
+      }
+      {
+        M3<TC> T = mul_dn_r2(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm));
+        T = mul_r2(T, u.template load<TC>(mu, xmm + opn));
+        axpy_su3_r2(G, c1, mul_r2(T, Um_xpn));
+      }
+      mul_nd_acc(S, G, u.template load<TC>(nu, xpm));
+    }
+    // ---- lower side: the paths that end in U_nu(x-nu+mu)
+    //   G = c0 U_nu(x-nu)^+ U_mu(x-nu)
+    //     + c1 U_nu(x-nu)^+ U_nu(x-2nu)^+ U_mu(x-2nu) U_nu(x-2nu+mu)
+    //     + c1 U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_mu(x-nu)
+    {
+      const M3<TC> Un_xmn = u.template load<TC>(nu, xmn);
+      const M3<TC> Um_xmn = u.template load<TC>(mu, xmn);
+      const M3<TC> P = mul_dn_r2(Un_xmn, Um_xmn);
+      // forward rectangle R2' = P U_mu(x-nu+mu) U_nu(x-nu+2mu)
+      add_su3_r2(R, mul_r2(mul_r2(P, u.template load<TC>(mu, xpm + omn)), u.template load<TC>(nu, xp2m + omn)));
+      M3<TC> G = zero3<TC>();
+      axpy_su3_r2(G, c0, P);
+      {
+        M3<TC> T = mul_dd_r2(Un_xmn, u.template load<TC>(nu, nv.at() + om2n));
+        T = mul_r2(T, u.template load<TC>(mu, nv.at() + om2n));
+        axpy_su3_r2(G, c1, mul_r2(T, u.template load<TC>(nu, xpm + om2n)));
+      }
+      {
+        M3<TC> T = mul_dd_r2(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm + omn));
+        T = mul_r2(T, u.template load<TC>(mu, xmm + omn));
+        axpy_su3_r2(G, c1, mul_r2(T, Um_xmn));
+      }
+      mul_acc(S, G, u.template load<TC>(nu, xpm + omn));
+    }
+  }
+}
+
+template <int ACT, class TC, class UV, class NAV>
+MAG_D M3<TC> staple_sum_nav(const UV& u, const NAV& nv, int mu, TC c0, TC c1) {
   M3<TC> S = zero3<TC>();
   M3<TC> R = zero3<TC>();
-  const int xpm = p.s + off_rt(g, p, mu, 1);
-  const int xmm = p.s + off_rt(g, p, mu, -1);
-  const int xp2m = p.s + off_rt(g, p, mu, 2);
+  const auto xpm = nv.at() + nv.off(mu, 1);
+  const auto xmm = nv.at() + nv.off(mu, -1);
+  const auto xp2m = nv.at() + nv.off(mu, 2);
 #pragma unroll 1
   for (int i = 0; i < 3; ++i) {
     const int nu = i + (i >= mu ? 1 : 0);
-    const int opn = off_rt(g, p, nu, 1), omn = off_rt(g, p, nu, -1);
-    const int xpn = p.s + opn, xmn = p.s + omn;
-    if (ACT == ACT_WILSON) {
-      // U_nu(x) U_mu(x+nu) U_nu(x+mu)^+
-      add_su3_r2(S, mul_nd_r2(mul_r2(u.template load_r2<TC>(nu, p.s), u.template load<TC>(mu, xpn)),
-                              u.template load<TC>(nu, xpm)));
-      // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu)
-      add_su3_r2(S, mul_r2(mul_dn_r2(u.template load<TC>(nu, xmn), u.template load<TC>(mu, xmn)),
-                           u.template load<TC>(nu, xpm + omn)));
-    } else {
-      const int op2n = off_rt(g, p, nu, 2), om2n = off_rt(g, p, nu, -2);
-      // ---- upper side of the plane: the paths that end in U_nu(x+mu)^+
-      //   G = c0 U_nu(x) U_mu(x+nu)                                (plaquette)
-      //     + c1 U_nu(x) U_nu(x+nu) U_mu(x+2nu) U_nu(x+mu+nu)^+    (tall rectangle)
-      //     + c1 U_mu(x-mu)^+ U_nu(x-mu) U_mu(x-mu+nu) U_mu(x+nu)  (backward rectangle)
-      // each an SU(3) product formed in two rows; S += G U_nu(x+mu)^+.
-      {
-        const M3<TC> Un_x = u.template load_r2<TC>(nu, p.s);
-        const M3<TC> Um_xpn = u.template load<TC>(mu, xpn);
-        const M3<TC> P = mul_r2(Un_x, Um_xpn);
-        // forward rectangle R1' = P U_mu(x+nu+mu) U_nu(x+2mu)^+
-        add_su3_r2(R, mul_nd_r2(mul_r2(P, u.template load<TC>(mu, xpm + opn)), u.template load<TC>(nu, xp2m)));
-        M3<TC> G = zero3<TC>();
-        axpy_su3_r2(G, c0, P);
-        {
-          M3<TC> T = mul_r2(Un_x, u.template load<TC>(nu, xpn));
-          T = mul_r2(T, u.template load<TC>(mu, p.s + op2n));
-          axpy_su3_r2(G, c1, mul_nd_r2(T, u.template load<TC>(nu, xpm + opn)));
-        }
-        {
-          M3<TC> T = mul_dn_r2(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm));
-          T = mul_r2(T, u.template load<TC>(mu, xmm + opn));
-          axpy_su3_r2(G, c1, mul_r2(T, Um_xpn));
-        }
-        mul_nd_acc(S, G, u.template load<TC>(nu, xpm));
-      }
-      // ---- lower side: the paths that end in U_nu(x-nu+mu)
-      //   G = c0 U_nu(x-nu)^+ U_mu(x-nu)
-      //     + c1 U_nu(x-nu)^+ U_nu(x-2nu)^+ U_mu(x-2nu) U_nu(x-2nu+mu)
-      //     + c1 U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_mu(x-nu)
-      {
-        const M3<TC> Un_xmn = u.template load<TC>(nu, xmn);
-        const M3<TC> Um_xmn = u.template load<TC>(mu, xmn);
-        const M3<TC> P = mul_dn_r2(Un_xmn, Um_xmn);
-        // forward rectangle R2' = P U_mu(x-nu+mu) U_nu(x-nu+2mu)
-        add_su3_r2(R, mul_r2(mul_r2(P, u.template load<TC>(mu, xpm + omn)), u.template load<TC>(nu, xp2m + omn)));
-        M3<TC> G = zero3<TC>();
-        axpy_su3_r2(G, c0, P);
-        {
-          M3<TC> T = mul_dd_r2(Un_xmn, u.template load<TC>(nu, p.s + om2n));
-          T = mul_r2(T, u.template load<TC>(mu, p.s + om2n));
-          axpy_su3_r2(G, c1, mul_r2(T, u.template load<TC>(nu, xpm + om2n)));
-        }
-        {
-          M3<TC> T = mul_dd_r2(u.template load<TC>(mu, xmm), u.template load<TC>(nu, xmm + omn));
-          T = mul_r2(T, u.template load<TC>(mu, xmm + omn));
-          axpy_su3_r2(G, c1, mul_r2(T, Um_xmn));
-        }
-        mul_acc(S, G, u.template load<TC>(nu, xpm + omn));
-      }
-    }
+    staple_plane<ACT, TC>(u, nv, mu, nu, xpm, xmm, xp2m, c0, c1, S, R);
   }
   if (ACT == ACT_IWASAKI) {
     R = scale(R, c1);
     mul_nd_acc(S, R, u.template load<TC>(mu, xpm));
   }
   return S;
+}
+
+template <int ACT, class TS, int NC, class TC>
+MAG_D M3<TC> staple_sum(const LinkView<TS, NC>& u, const Geom& g, const Site& p, int mu, TC c0, TC c1) {
+  return staple_sum_nav<ACT, TC>(u, GNav{g, p}, mu, c0, c1);
 }
 
 // Output of a force evaluation (chosen at run time; wave-uniform).
@@ -446,6 +472,140 @@ __global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top(StoutTopArg
 template <int ACT, class TS, int NC, class TC, int NCN = NC>
 __global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top_b(const StoutTopArgs<TS, NC, NCN>* t) {
   k_stout_top_body<ACT, TS, NC, TC, NCN>(t[blockIdx.z]);
+}
+
+// ---- LDS-tiled variant of the FP32 Iwasaki top kernel (MAG_TOP_LDS).
+// A block's 256 sites are whole x0 rows of one (x2, x3) slice. The block
+// first stages the x0 and x1 links of its rows and of two halo rows on
+// either side (256 + 4 L0 sites, at most TILE_NT) in LDS, either as the 12
+// stored reals or rebuilt to 18 (padded to 20, five 16-byte groups). The
+// staple terms of the (x0, x1) plane (mu and nu both 0 or 1) then read all
+// their links from LDS; the other planes read global memory.
+constexpr int TILE_NT = 384;
+
+// Shapes the tile covers: L0 a power of two <= 32, L1 a power of two, and
+// whole-block slices.
+MAG_HD bool top_tile_ok(const Geom& g) {
+  const int L0 = g.L[0], L1 = g.L[1];
+  return (L0 == 4 || L0 == 8 || L0 == 16 || L0 == 32) && L1 > 0 && (L1 & (L1 - 1)) == 0 && (L0 * L1) % BLOCK == 0;
+}
+
+// A site of the block's slice within two rows of its own is at tile index
+// (s - tb) mod (L0 L1), tb = the global index of the first staged site
+// (tile rows follow x1 cyclically, so the power-of-two mask also covers the
+// rows that wrap round the lattice).
+template <int NCL>
+struct TileLinks {
+  using VT = PackVec<float, 4>::type;
+  static constexpr int NG = NCL / 4;
+  const VT* lds;  // [direction 0, 1][group][TILE_NT]
+  int tb, pm;     // first staged site, L0 L1 - 1
+  template <class TC, bool FULL>
+  MAG_D M3<TC> get(int dir, int s) const {
+    const int t = (s - tb) & pm;
+    constexpr int NR = (FULL && NCL == 20) ? 5 : 3;
+    float r[4 * NR];
+#pragma unroll
+    for (int gi = 0; gi < NR; ++gi) {
+      const VT v = lds[(dir * NG + gi) * TILE_NT + t];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[4 * gi + k] = v[k];
+    }
+    M3<TC> m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) m.e[k] = {TC(r[2 * k]), TC(r[2 * k + 1])};
+    if constexpr (FULL && NCL == 20) {
+#pragma unroll
+      for (int k = 6; k < 9; ++k) m.e[k] = {TC(r[2 * k]), TC(r[2 * k + 1])};
+    } else if constexpr (FULL) {
+      reconstruct_row2(m);
+    } else {
+#pragma unroll
+      for (int k = 6; k < 9; ++k) m.e[k] = {TC(0), TC(0)};
+    }
+    return m;
+  }
+};
+
+// The staged links as a link view (directions 0 and 1, sites of the tile).
+template <int NCL>
+struct TileView {
+  TileLinks<NCL> t;
+  template <class TC>
+  MAG_D M3<TC> load(int dir, int s) const { return t.template get<TC, true>(dir, s); }
+  template <class TC>
+  MAG_D M3<TC> load_r2(int dir, int s) const { return t.template get<TC, false>(dir, s); }
+};
+
+template <int NCL>
+MAG_D void k_stout_top_tile_body(const StoutTopArgs<float, 12, 12>& a) {
+  using VT = PackVec<float, 4>::type;
+  constexpr int NG = NCL / 4;
+  __shared__ VT tile[2 * NG * TILE_NT];
+  int mu, sblk;
+  stencil_tile<false>(a.g, mu, sblk);
+  const int L0 = a.g.L[0], L1 = a.g.L[1], plane = L0 * L1;
+  const int s0 = sblk * BLOCK;
+  const int sl = s0 - s0 % plane;  // first site of the (x2, x3) slice
+  const int x1lo = (s0 - sl) / L0;
+  const int nt = BLOCK + 4 * L0;
+  for (int ti = threadIdx.x; ti < nt; ti += BLOCK) {
+    const int r = ti / L0, x0 = ti - r * L0;
+    int x1 = (x1lo - 2 + r) % L1;
+    if (x1 < 0) x1 += L1;
+    const unsigned gs = (unsigned)(sl + x1 * L0 + x0);
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      if constexpr (NCL == 12) {
+#pragma unroll
+        for (int gi = 0; gi < 3; ++gi)
+          tile[(d * NG + gi) * TILE_NT + ti] = at(Packed<float, 12>::group(a.Un.p, a.Un.V, d, gi), gs);
+      } else {
+        const M3<float> m = a.Un.template load<float>(d, (int)gs);
+#pragma unroll
+        for (int gi = 0; gi < 5; ++gi) {
+          VT v;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int c = 4 * gi + k;
+            v[k] = c < 18 ? ((c & 1) ? m.e[c / 2].i : m.e[c / 2].r) : 0.f;
+          }
+          tile[(d * NG + gi) * TILE_NT + ti] = v;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int s = s0 + (int)threadIdx.x;
+  const float c1 = float(a.c1);
+  const float c0 = 1.f - 8.f * c1;
+  const Site p = site_of(a.g, s);
+  const GNav nv{a.g, p};
+  const TileView<NCL> tv{TileLinks<NCL>{tile, sl + (x1lo - 2) * L0, plane - 1}};
+  // the (x0, x1) plane from LDS, the other planes from global memory
+  M3<float> S = zero3<float>();
+  M3<float> R = zero3<float>();
+  const int xpm = s + off_rt(a.g, p, mu, 1), xmm = s + off_rt(a.g, p, mu, -1), xp2m = s + off_rt(a.g, p, mu, 2);
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const int nu = i + (i >= mu ? 1 : 0);
+    if (mu < 2 && nu < 2) staple_plane<ACT_IWASAKI, float>(tv, nv, mu, nu, xpm, xmm, xp2m, c0, c1, S, R);
+    else staple_plane<ACT_IWASAKI, float>(a.Un, nv, mu, nu, xpm, xmm, xp2m, c0, c1, S, R);
+  }
+  R = scale(R, c1);
+  mul_nd_acc(S, R, a.Un.template load<float>(mu, xpm));
+  const M3<float> Z = scale(dag(S), float(-a.beta / 3.));
+  stout_lambda_link<float, 12, float>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
+}
+template <int NCL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(NCL == 12 ? 3 : 2))) void k_stout_top_tile(
+    StoutTopArgs<float, 12, 12> a) {
+  k_stout_top_tile_body<NCL>(a);
+}
+template <int NCL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(NCL == 12 ? 3 : 2))) void k_stout_top_tile_b(
+    const StoutTopArgs<float, 12, 12>* t) {
+  k_stout_top_tile_body<NCL>(t[blockIdx.z]);
 }
 
 // Backward level k (eq. 75): Z_k = W_k + sum over nu of the six

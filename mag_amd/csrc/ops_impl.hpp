@@ -9,6 +9,12 @@
 #include "launch.hpp"
 #include "runtime.hpp"
 
+// Reals per staged link of the LDS-tiled FP32 Iwasaki top kernel (12 or 20;
+// 0: the untiled kernel).
+#ifndef MAG_TOP_LDS
+#define MAG_TOP_LDS 0
+#endif
+
 namespace mag {
 
 template <class TS, int NC>
@@ -98,12 +104,21 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
                                  act.beta,
                                  act.c1,
                                  act.rho};
-    if (iw)
-      r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS, NCN>,
-                 k_stout_top_b<ACT_IWASAKI, TS, NC, TS, NCN>, grid_stencil(g), tp);
-    else
-      r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS, NCN>, k_stout_top_b<ACT_WILSON, TS, NC, TS, NCN>,
-                 grid_stencil(g), tp);
+    bool tiled = false;  // the LDS-tiled FP32 Iwasaki top kernel, when compiled in and the shape fits
+    if constexpr (MAG_TOP_LDS != 0 && std::is_same<TS, float>::value && NC == 12) {
+      tiled = iw && top_tile_ok(g);
+      if (tiled)
+        r = launch(c, "stout_top", k_stout_top_tile<MAG_TOP_LDS>, k_stout_top_tile_b<MAG_TOP_LDS>, grid_stencil(g),
+                   tp);
+    }
+    if (!tiled) {
+      if (iw)
+        r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS, NCN>,
+                   k_stout_top_b<ACT_IWASAKI, TS, NC, TS, NCN>, grid_stencil(g), tp);
+      else
+        r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS, NCN>,
+                   k_stout_top_b<ACT_WILSON, TS, NC, TS, NCN>, grid_stencil(g), tp);
+    }
     if (r) return r;
   }
   for (int k = n - 1; k >= 1; --k) {
