@@ -97,6 +97,17 @@ struct GNav {
   MAG_D int off(int mu, int d) const { return off_rt(g, p, mu, d); }
 };
 
+// A view of an SU(3) link field whose loads read 18-real links as rows 0 and
+// 1 and rebuild row 2 (two thirds of the groups).
+template <class TS, int NC>
+struct SU3Read {
+  LinkView<TS, NC> u;
+  template <class TC>
+  MAG_D M3<TC> load(int mu, int s) const { return u.template load_su3<TC, true>(mu, s); }
+  template <class TC>
+  MAG_D M3<TC> load_r2(int mu, int s) const { return u.template load_r2<TC>(mu, s); }
+};
+
 // Undaggered staple sum S of link (x, mu): c0 * (6 plaquette staples)
 // + c1 * (18 rectangle staples), the paths of get_staple_dagger
 // (field-hmc.h:226-240) and get_rectangular_dagger (field-hmc.h:137-201),
@@ -387,15 +398,17 @@ struct StoutFwdArgs {
   int store_qc;  // 0 when only the smeared field is wanted (Hamiltonian)
 };
 
-template <class TSI, int NCI, class TSO, int NCO, class TC>
+template <class TSI, int NCI, class TSO, int NCO, class TC, bool R12 = false>
 MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
   int mu, sblk;
   stencil_tile(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const Site p = site_of(a.g, s);
-  const M3<TC> S = staple_sum<ACT_WILSON, TSI, NCI, TC>(a.U, a.g, p, mu, TC(1), TC(0));
-  const M3<TC> Ux = a.U.template load<TC>(mu, s);
+  M3<TC> S;
+  if constexpr (R12) S = staple_sum_nav<ACT_WILSON, TC>(SU3Read<TSI, NCI>{a.U}, GNav{a.g, p}, mu, TC(1), TC(0));
+  else S = staple_sum<ACT_WILSON, TSI, NCI, TC>(a.U, a.g, p, mu, TC(1), TC(0));
+  const M3<TC> Ux = a.U.template load_su3<TC, R12>(mu, s);
   TC q[8];
   force_alg_from(mul_nd(S, Ux), q);  // -i TA(S U^+)
   const TC rho = TC(a.rho);
@@ -408,19 +421,19 @@ MAG_D void k_stout_fwd_body(const StoutFwdArgs<TSI, NCI, TSO, NCO>& a) {
     a.C.store_nt(mu, s, scale(S, rho));
   }
 }
-template <class TSI, int NCI, class TSO, int NCO, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(a); }
-template <class TSI, int NCI, class TSO, int NCO, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd_b(const StoutFwdArgs<TSI, NCI, TSO, NCO>* t) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC>(t[blockIdx.z]); }
+template <class TSI, int NCI, class TSO, int NCO, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd(StoutFwdArgs<TSI, NCI, TSO, NCO> a) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC, R12>(a); }
+template <class TSI, int NCI, class TSO, int NCO, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_fwd_b(const StoutFwdArgs<TSI, NCI, TSO, NCO>* t) { k_stout_fwd_body<TSI, NCI, TSO, NCO, TC, R12>(t[blockIdx.z]); }
 
 // Lambda step of level k for one link, given Z' = dS/dU_{k+1} there
 // (compute_Lambda, field-multi.h:180-269, eqs. 72-73):
 //   Lambda_k = HT(Gamma)   (stored as its 8 generator components),
 //   W_k = Z' e^{iQ_k} + i C_k^+ Lambda_k   (the local part of Z_k, eq. 75).
-template <class TS, int NC, class TC>
+template <class TS, int NC, class TC, bool R12 = false>
 MAG_D void stout_lambda_link(int mu, int s, const M3<TC>& Zp, const LinkView<TS, NC>& U, const AlgView<TS>& Qv,
                              const MatView<TS>& Cv, const AlgView<TS>& Lout, const MatView<TS>& Wout) {
-  const M3<TC> Uk = U.template load_nt<TC>(mu, s);
+  const M3<TC> Uk = U.template load_su3_nt<TC, R12>(mu, s);
   const M3<TC> Q = Qv.template load_mat_nt<TC>(mu, s);
   const M3<TC> X = mul(Uk, Zp);
   TC lam[8];
@@ -664,7 +677,7 @@ MAG_HD void mul_nd_sub(M3<T>& c, const M3<T>& a, const M3<T>& b) {
     }
 }
 
-template <class TS, int NC, class TC>
+template <class TS, int NC, class TC, bool R12>
 MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int mu) {
   const Geom& g = a.g;
   M3<TC> acc = zero3<TC>();
@@ -676,18 +689,18 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
     const int xpn = p.s + opn, xmn = p.s + omn, xpmmn = xpm + omn;
     {
       // D and E = D U_nu(x)^+ are products of links: formed in two rows, completed once
-      M3<TC> D = mul_nd_r2(a.U.template load_r2<TC>(nu, xpm), a.U.template load<TC>(mu, xpn));
+      M3<TC> D = mul_nd_r2(a.U.template load_r2<TC>(nu, xpm), a.U.template load_su3<TC, R12>(mu, xpn));
       reconstruct_row2(D);
       M3<TC> G = mul_hn(a.L.template load_mat<TC>(nu, xpm), D);
       mul_sub_nh(G, D, a.L.template load_mat<TC>(mu, xpn));
-      const M3<TC> Un_x = a.U.template load<TC>(nu, p.s);
+      const M3<TC> Un_x = a.U.template load_su3<TC, R12>(nu, p.s);
       mul_nd_acc(acc, G, Un_x);
       M3<TC> E = mul_nd_r2(D, Un_x);
       reconstruct_row2(E);
       mul_sub_nh(acc, E, a.L.template load_mat<TC>(nu, p.s));
     }
     {
-      const M3<TC> Um_xmn = a.U.template load<TC>(mu, xmn);
+      const M3<TC> Um_xmn = a.U.template load_su3<TC, R12>(mu, xmn);
       TC ln[8], lm[8];
       a.L.load(nu, xmn, ln);
       a.L.load(mu, xmn, lm);
@@ -695,8 +708,8 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
       for (int k = 0; k < 8; ++k) ln[k] -= lm[k];
       M3<TC> K = mul_dh(Um_xmn, alg_to_mat(ln));
       mul_hd_sub(K, a.L.template load_mat<TC>(nu, xpmmn), Um_xmn);
-      const M3<TC> K2 = mul_dn(a.U.template load<TC>(nu, xpmmn), K);
-      mul_acc(acc, K2, a.U.template load<TC>(nu, xmn));
+      const M3<TC> K2 = mul_dn(a.U.template load_su3<TC, R12>(nu, xpmmn), K);
+      mul_acc(acc, K2, a.U.template load_su3<TC, R12>(nu, xmn));
     }
   }
   // Z = W + i rho acc
@@ -707,29 +720,29 @@ MAG_D M3<TC> stout_six_terms(const StoutBwdArgs<TS, NC>& a, const Site& p, int m
   return Z;
 }
 
-template <bool LAST, class TS, int NC, class TC>
+template <bool LAST, class TS, int NC, class TC, bool R12 = false>
 MAG_D void k_stout_bwd_body(const StoutBwdArgs<TS, NC>& a) {
   int mu, sblk;
   stencil_tile<sizeof(TC) == sizeof(double)>(a.g, mu, sblk);
   const int s = sblk * BLOCK + threadIdx.x;
   if (s >= a.g.V) return;
   const Site p = site_of(a.g, s);
-  const M3<TC> Z = stout_six_terms<TS, NC, TC>(a, p, mu);
+  const M3<TC> Z = stout_six_terms<TS, NC, TC, R12>(a, p, mu);
   if (!LAST) {
-    stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
+    stout_lambda_link<TS, NC, TC, R12>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
   } else if (a.f.mag) {
     a.Zout.store(mu, s, Z);
   } else {
-    const M3<TC> Ux = a.U.template load<TC>(mu, s);
+    const M3<TC> Ux = a.U.template load_su3<TC, R12>(mu, s);
     TC f[8];
     force_alg_from(mul(Ux, Z), f);
     force_epilogue<TS, NC, TC>(a.f, mu, s, Ux, f);
   }
 }
-template <bool LAST, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_bwd(StoutBwdArgs<TS, NC> a) { k_stout_bwd_body<LAST, TS, NC, TC>(a); }
-template <bool LAST, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_bwd_b(const StoutBwdArgs<TS, NC>* t) { k_stout_bwd_body<LAST, TS, NC, TC>(t[blockIdx.z]); }
+template <bool LAST, class TS, int NC, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_bwd(StoutBwdArgs<TS, NC> a) { k_stout_bwd_body<LAST, TS, NC, TC, R12>(a); }
+template <bool LAST, class TS, int NC, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_bwd_b(const StoutBwdArgs<TS, NC>* t) { k_stout_bwd_body<LAST, TS, NC, TC, R12>(t[blockIdx.z]); }
 
 // Constrained runs with smearing: F from a stored Z_0 with the classes of
 // field-hmc.h:353-378: W = U Z (0), U(x+mu) Z(x+mu) - Z U (1, 10), 0 (100);

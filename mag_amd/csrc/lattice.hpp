@@ -241,6 +241,16 @@ struct LinkView {
       return load<TC>(mu, s);
     }
   }
+  template <class TC, bool R12>
+  MAG_D M3<TC> load_su3_nt(int mu, int s) const {
+    if constexpr (R12 && NC == 18) {
+      M3<TC> m = load_r2_nt<TC>(mu, s);
+      reconstruct_row2(m);
+      return m;
+    } else {
+      return get<TC, true>(mu, s);
+    }
+  }
   template <class TC>
   MAG_D void store(int mu, int s, const M3<TC>& m) const { put<TC, false>(mu, s, m); }
   // Non-temporal forms for data a kernel touches once, at its own link

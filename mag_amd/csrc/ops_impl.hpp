@@ -15,7 +15,19 @@
 #define MAG_TOP_LDS 0
 #endif
 
+// FP64 18-real links read as rows 0 and 1 with row 2 rebuilt by the kernels
+// of the smeared force that read them most (stout forward and backward
+// levels) and by the link update; the top kernel and the thin-link force
+// read all 18 reals (rebuilding spills there).
+#ifndef MAG_FP64_READ12
+#define MAG_FP64_READ12 15
+#endif
+
 namespace mag {
+
+// whether Ops<TS, NC> reads its links as two rows in those kernels
+template <class TS, int NC>
+constexpr bool ops_read12(int bit) { return (MAG_FP64_READ12 & bit) != 0 && std::is_same<TS, double>::value && NC == 18; }
 
 template <class TS, int NC>
 LinkView<TS, NC> lview(void* p, const Geom& g) { return LinkView<TS, NC>{(TS*)p, g.V}; }
@@ -27,7 +39,8 @@ MatView<TS> mview(void* p, const Geom& g) { return MatView<TS>{(TS*)p, g.V}; }
 template <class TS, int NC>
 int Ops<TS, NC>::link_update(mag_ctx c, const Geom& g, void* Uin, void* Uout, void* P, double eps, int mag) {
   UpdateArgs<TS, NC> a{g, lview<TS, NC>(Uin, g), lview<TS, NC>(Uout, g), aview<TS>(P, g), eps, mag};
-  return launch(c, "link_update", k_link_update<TS, NC, TS>, k_link_update_b<TS, NC, TS>, grid_links(g), a);
+  constexpr bool R12 = ops_read12<TS, NC>(8);
+  return launch(c, "link_update", k_link_update<TS, NC, TS, R12>, k_link_update_b<TS, NC, TS, R12>, grid_links(g), a);
 }
 
 // Force of the (possibly smeared) action on U with one of the epilogues
@@ -72,6 +85,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
                   k_gauge_force_b<ACT_WILSON, false, TS, NC, TS>, grid_stencil(g), fa);
   }
   const int n = act.n_stout;
+  constexpr bool R12 = ops_read12<TS, NC>(1), R12B = ops_read12<TS, NC>(2), R12L = ops_read12<TS, NC>(4);
   if ((r = ws->need_stout(n))) return r;
   // FP64 12-real chains keep the top level U_n as 18 reals (written by the
   // last forward level, read only by the top kernel's staple sum).
@@ -87,7 +101,7 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     } else {
       StoutFwdArgs<TS, NC, TS, NC> fw{g, lview<TS, NC>(Ulev(k), g), aview<TS>(ws->Qs[k], g),
                                       mview<TS>(ws->Cs[k], g), lview<TS, NC>(ws->Us[k], g), act.rho, 1};
-      r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS>, k_stout_fwd_b<TS, NC, TS, NC, TS>,
+      r = launch(c, "stout_fwd", k_stout_fwd<TS, NC, TS, NC, TS, R12>, k_stout_fwd_b<TS, NC, TS, NC, TS, R12>,
                  grid_stencil(g), fw);
     }
     if (r) return r;
@@ -133,8 +147,8 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     bw.Lout = aview<TS>(ws->L[1 - cur], g);
     bw.Wout = mview<TS>(ws->W, g);
     bw.rho = act.rho;
-    if ((r = launch(c, "stout_bwd", k_stout_bwd<false, TS, NC, TS>, k_stout_bwd_b<false, TS, NC, TS>, grid_stencil(g),
-                    bw)))
+    if ((r = launch(c, "stout_bwd", k_stout_bwd<false, TS, NC, TS, R12B>, k_stout_bwd_b<false, TS, NC, TS, R12B>,
+                    grid_stencil(g), bw)))
       return r;
     cur = 1 - cur;
   }
@@ -149,8 +163,8 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
     if ((r = ws->need_Z())) return r;
     bw.Zout = mview<TS>(ws->Z, g);
   }
-  if ((r = launch(c, "stout_bwd_force", k_stout_bwd<true, TS, NC, TS>, k_stout_bwd_b<true, TS, NC, TS>, grid_stencil(g),
-                  bw)))
+  if ((r = launch(c, "stout_bwd_force", k_stout_bwd<true, TS, NC, TS, R12L>, k_stout_bwd_b<true, TS, NC, TS, R12L>,
+                  grid_stencil(g), bw)))
     return r;
   if (!act.mag) return MAG_OK;
   // constrained: F from the stored Z_0 with the link classes
