@@ -205,9 +205,10 @@ int mag_chain_reset_demon(mag_chain c);
 /* Run ntraj trajectories; results[i] for each (may be NULL). Trajectory
  * indices continue across calls (they key the momentum and die-roll streams). */
 int mag_chain_run(mag_chain c, int ntraj, mag_traj_result* results);
-/* FP64 reversibility check from the current state: trajectory forward,
- * momenta flipped, trajectory back. Returns max |U_fwd->bwd - U| and the
- * Hamiltonian change of the round trip; the chain state is unchanged. */
+/* FP64 reversibility check from the current state (its links widened to FP64
+ * and projected onto SU(3)): trajectory forward, momenta flipped, trajectory
+ * back. Returns max |U_fwd->bwd - U| and the Hamiltonian change of the round
+ * trip; the chain state is unchanged. */
 int mag_chain_reversibility(mag_chain c, double* max_link_dev, double* dh_roundtrip);
 int mag_chain_demon_state(mag_chain c, double state[4]); /* S0p, S0r, Ep, Er */
 int mag_chain_traj_index(mag_chain c, long* next_traj);

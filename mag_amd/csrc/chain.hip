@@ -278,7 +278,10 @@ int mag_chain_traj_index(mag_chain c, long* next) {
   return MAG_OK;
 }
 
-// FP64 round trip from the current state: momenta from a stream epoch that no
+// FP64 round trip from the current state: the links widened to FP64 and
+// projected onto SU(3) (FP32 links are off the FP64 group by ~1e-7, and the
+// FP64 link update reads links as two rows, which would project them in its
+// first step and not in the step back), momenta from a stream epoch that no
 // trajectory uses (2^40 + trajectory index), FG forward, momenta flipped, FG
 // forward again, momenta flipped back. The chain's own fields are untouched.
 int mag_chain_reversibility(mag_chain c, double* max_link_dev, double* dh_roundtrip) {
@@ -289,12 +292,14 @@ int mag_chain_reversibility(mag_chain c, double* max_link_dev, double* dh_roundt
                   (r = mag_field_create(x, c->dims, MAG_LINK18, MAG_F64, &c->U64b)) ||
                   (r = mag_field_create(x, c->dims, MAG_ALG8, MAG_F64, &c->P64))))
     return r;
-  if ((r = mag_field_copy(c->U64, c->U)) || (r = mag_field_copy(c->U64b, c->U64))) return r;
+  using O = Ops<double, 18>;
+  if ((r = mag_field_copy(c->U64, c->U)) || (r = O::reunitarize(x, c->g, c->U64->p, 0)) ||
+      (r = mag_field_copy(c->U64b, c->U64)))
+    return r;
   const uint64_t epoch = (1ull << 40) + (uint64_t)c->traj;
   if ((r = mom_refresh(x, c->g, MAG_F64, c->P64->p, c->p.seed, epoch, 1.))) return r;
   double parts[2], h0 = 0., h1 = 0.;
   if ((r = mag_hamiltonian(c->U64, c->P64, &c->act, parts, &h0))) return r;
-  using O = Ops<double, 18>;
   Workspace* ws = workspace(x, c->g, MAG_F64, 18);
   const int n = c->p.trajectory_length;
   const double dt = c->p.dt;

@@ -4,7 +4,8 @@ integrator with tau = 1 and 12 steps, FP64 Hamiltonian and accept with the
 demon window held at +-0.5 % of the initial action. 100 trajectories from a
 start thermalised by 30 forced-accept trajectories (Haar seed 33), with an
 FP64 reversibility round trip (forward trajectory,
-momentum flip, forward again, from the FP32 state promoted to FP64) after
+momentum flip, forward again, from the FP32 state widened to FP64 and
+projected onto SU(3)) after
 every 10: links back to 1e-10, Hamiltonian back to 1e-6 of a total of ~3e7.
 Over the run: acceptance, and <exp(-dH)> = 1 within four standard errors."""
 import numpy as np
