@@ -465,7 +465,10 @@ struct StoutTopArgs {
   double beta, c1, rho;
 };
 
-template <int ACT, class TS, int NC, class TC, int NCN>
+// R12: the staple links of U_n (and U_{n-1}) read as rows 0 and 1 with row 2
+// rebuilt; such a kernel runs at one wave per SIMD (its spills then go to
+// AGPRs, not scratch).
+template <int ACT, class TS, int NC, class TC, int NCN, bool R12 = false>
 MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC, NCN>& a) {
   int mu, sblk;
   stencil_tile<sizeof(TC) == sizeof(double)>(a.g, mu, sblk);
@@ -474,17 +477,23 @@ MAG_D void k_stout_top_body(const StoutTopArgs<TS, NC, NCN>& a) {
   const TC c1 = (ACT == ACT_IWASAKI) ? TC(a.c1) : TC(0);
   const TC c0 = TC(1) - TC(8) * c1;
   const Site p = site_of(a.g, s);
-  const M3<TC> S = staple_sum<ACT, TS, NCN, TC>(a.Un, a.g, p, mu, c0, c1);
+  M3<TC> S;
+  if constexpr (R12) S = staple_sum_nav<ACT, TC>(SU3Read<TS, NCN>{a.Un}, GNav{a.g, p}, mu, c0, c1);
+  else S = staple_sum<ACT, TS, NCN, TC>(a.Un, a.g, p, mu, c0, c1);
   const M3<TC> Z = scale(dag(S), TC(-a.beta / 3.));
-  stout_lambda_link<TS, NC, TC>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
+  stout_lambda_link<TS, NC, TC, R12>(mu, s, Z, a.Ulo, a.Qlo, a.Clo, a.Lout, a.Wout);
 }
-template <int ACT, class TS, int NC, class TC, int NCN = NC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top(StoutTopArgs<TS, NC, NCN> a) {
-  k_stout_top_body<ACT, TS, NC, TC, NCN>(a);
+template <class TC, bool R12>
+constexpr int top_waves() { return R12 ? 1 : stencil_waves<TC>(); }
+template <int ACT, class TS, int NC, class TC, int NCN = NC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(top_waves<TC, R12>()))) void k_stout_top(
+    StoutTopArgs<TS, NC, NCN> a) {
+  k_stout_top_body<ACT, TS, NC, TC, NCN, R12>(a);
 }
-template <int ACT, class TS, int NC, class TC, int NCN = NC>
-__global__ __launch_bounds__(BLOCK) MAG_STENCIL_OCC void k_stout_top_b(const StoutTopArgs<TS, NC, NCN>* t) {
-  k_stout_top_body<ACT, TS, NC, TC, NCN>(t[blockIdx.z]);
+template <int ACT, class TS, int NC, class TC, int NCN = NC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(top_waves<TC, R12>()))) void k_stout_top_b(
+    const StoutTopArgs<TS, NC, NCN>* t) {
+  k_stout_top_body<ACT, TS, NC, TC, NCN, R12>(t[blockIdx.z]);
 }
 
 // ---- LDS-tiled variant of the FP32 Iwasaki top kernel (MAG_TOP_LDS).

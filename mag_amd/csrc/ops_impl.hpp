@@ -15,10 +15,10 @@
 #define MAG_TOP_LDS 0
 #endif
 
-// FP64 18-real links read as rows 0 and 1 with row 2 rebuilt by the kernels
-// of the smeared force that read them most (stout forward and backward
-// levels) and by the link update; the top kernel and the thin-link force
-// read all 18 reals (rebuilding spills there).
+// FP64 18-real links read as rows 0 and 1 with row 2 rebuilt, a bit per
+// kernel: 1 stout forward, 2 stout backward, 4 last backward, 8 link update
+// (on by default), 16 the top kernel at one wave per SIMD (off: as fast as
+// its full reads at two waves). The thin-link force reads all 18 reals.
 #ifndef MAG_FP64_READ12
 #define MAG_FP64_READ12 15
 #endif
@@ -125,10 +125,12 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
         r = launch(c, "stout_top", k_stout_top_tile<MAG_TOP_LDS>, k_stout_top_tile_b<MAG_TOP_LDS>, grid_stencil(g),
                    tp);
     }
+    // FP64 (both link formats: the top level U_n is stored as 18 reals)
+    constexpr bool R12T = (MAG_FP64_READ12 & 16) != 0 && std::is_same<TS, double>::value;
     if (!tiled) {
       if (iw)
-        r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS, NCN>,
-                   k_stout_top_b<ACT_IWASAKI, TS, NC, TS, NCN>, grid_stencil(g), tp);
+        r = launch(c, "stout_top", k_stout_top<ACT_IWASAKI, TS, NC, TS, NCN, R12T>,
+                   k_stout_top_b<ACT_IWASAKI, TS, NC, TS, NCN, R12T>, grid_stencil(g), tp);
       else
         r = launch(c, "stout_top", k_stout_top<ACT_WILSON, TS, NC, TS, NCN>,
                    k_stout_top_b<ACT_WILSON, TS, NC, TS, NCN>, grid_stencil(g), tp);
