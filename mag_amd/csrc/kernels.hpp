@@ -260,7 +260,7 @@ MAG_D void force_epilogue(const ForceArgs<TS, NC>& a, int mu, int s, const M3<TC
 // 10, and F = 0 for class 100. Stored as F_a = -(beta/3) Im Tr(T_a W).
 // CONS = false (mag = 0) compiles the free-link form alone (one staple sum,
 // no class loop, U(x) loaded after it).
-template <int ACT, bool CONS, class TS, int NC, class TC>
+template <int ACT, bool CONS, class TS, int NC, class TC, bool R12 = false>
 MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   int mu, sblk;
   stencil_tile<sizeof(TC) == sizeof(double)>(a.g, mu, sblk);
@@ -272,12 +272,14 @@ MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   M3<TC> Ux, W = zero3<TC>();
   if (!CONS) {
     // U(x) is loaded after the staple sum, so it is not live across it
-    const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, p, mu, c0, c1);
-    Ux = a.U.template load<TC>(mu, s);
+    M3<TC> S;
+    if constexpr (R12) S = staple_sum_nav<ACT, TC>(SU3Read<TS, NC>{a.U}, GNav{a.g, p}, mu, c0, c1);
+    else S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, p, mu, c0, c1);
+    Ux = a.U.template load_su3<TC, R12>(mu, s);
     W = mul_nd(Ux, S);
   } else {
     const int cls = link_class_rt(p, mu, a.mag);
-    Ux = a.U.template load<TC>(mu, s);
+    Ux = a.U.template load_su3<TC, R12>(mu, s);
     if (cls != 100) {
       // class 0: one staple at x; classes 1, 10: staples at x and x+mu
       const int nsite = (cls == 0) ? 1 : 2;
@@ -285,10 +287,12 @@ MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
 #pragma unroll 1
       for (int k = 0; k < nsite; ++k) {
         const Site q = (k == 0) ? p : site_of(a.g, ys);
-        const M3<TC> S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, q, mu, c0, c1);
+        M3<TC> S;
+        if constexpr (R12) S = staple_sum_nav<ACT, TC>(SU3Read<TS, NC>{a.U}, GNav{a.g, q}, mu, c0, c1);
+        else S = staple_sum<ACT, TS, NC, TC>(a.U, a.g, q, mu, c0, c1);
         if (cls == 0) W = mul_nd(Ux, S);
         else if (k == 0) W = sub(W, mul_dn(S, Ux));
-        else mul_nd_acc(W, a.U.template load<TC>(mu, ys), S);
+        else mul_nd_acc(W, a.U.template load_su3<TC, R12>(mu, ys), S);
       }
     }
   }
@@ -299,10 +303,10 @@ MAG_D void k_gauge_force_body(const ForceArgs<TS, NC>& a) {
   for (int k = 0; k < 8; ++k) f[k] *= fac;
   force_epilogue<TS, NC, TC>(a, mu, s, Ux, f);
 }
-template <int ACT, bool CONS, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_GFORCE_OCC void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, CONS, TS, NC, TC>(a); }
-template <int ACT, bool CONS, class TS, int NC, class TC>
-__global__ __launch_bounds__(BLOCK) MAG_GFORCE_OCC void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, CONS, TS, NC, TC>(t[blockIdx.z]); }
+template <int ACT, bool CONS, class TS, int NC, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) MAG_GFORCE_OCC void k_gauge_force(ForceArgs<TS, NC> a) { k_gauge_force_body<ACT, CONS, TS, NC, TC, R12>(a); }
+template <int ACT, bool CONS, class TS, int NC, class TC, bool R12 = false>
+__global__ __launch_bounds__(BLOCK) MAG_GFORCE_OCC void k_gauge_force_b(const ForceArgs<TS, NC>* t) { k_gauge_force_body<ACT, CONS, TS, NC, TC, R12>(t[blockIdx.z]); }
 
 // ============================================================ link updates
 // evolve_gauge_field (field-hmc.h:442-482) with the constraint classes:

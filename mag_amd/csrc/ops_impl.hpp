@@ -18,7 +18,7 @@
 // FP64 18-real links read as rows 0 and 1 with row 2 rebuilt, a bit per
 // kernel: 1 stout forward, 2 stout backward, 4 last backward, 8 link update
 // (on by default), 16 the top kernel at one wave per SIMD (off: as fast as
-// its full reads at two waves). The thin-link force reads all 18 reals.
+// its full reads at two waves), 32 the thin-link force (off: within 1 %).
 #ifndef MAG_FP64_READ12
 #define MAG_FP64_READ12 15
 #endif
@@ -72,17 +72,18 @@ int Ops<TS, NC>::force(mag_ctx c, const Geom& g, void* U, const mag_action& act,
   fa.eps = eps;
   const bool iw = act.type == MAG_IWASAKI;
   if (act.n_stout <= 0) {
+    constexpr bool R12 = ops_read12<TS, NC>(32);
     if (iw && act.mag)
-      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, true, TS, NC, TS>,
-                    k_gauge_force_b<ACT_IWASAKI, true, TS, NC, TS>, grid_stencil(g), fa);
+      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, true, TS, NC, TS, R12>,
+                    k_gauge_force_b<ACT_IWASAKI, true, TS, NC, TS, R12>, grid_stencil(g), fa);
     if (iw)
-      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, false, TS, NC, TS>,
-                    k_gauge_force_b<ACT_IWASAKI, false, TS, NC, TS>, grid_stencil(g), fa);
+      return launch(c, "force_iwasaki", k_gauge_force<ACT_IWASAKI, false, TS, NC, TS, R12>,
+                    k_gauge_force_b<ACT_IWASAKI, false, TS, NC, TS, R12>, grid_stencil(g), fa);
     if (act.mag)
-      return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, true, TS, NC, TS>,
-                    k_gauge_force_b<ACT_WILSON, true, TS, NC, TS>, grid_stencil(g), fa);
-    return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, false, TS, NC, TS>,
-                  k_gauge_force_b<ACT_WILSON, false, TS, NC, TS>, grid_stencil(g), fa);
+      return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, true, TS, NC, TS, R12>,
+                    k_gauge_force_b<ACT_WILSON, true, TS, NC, TS, R12>, grid_stencil(g), fa);
+    return launch(c, "force_wilson", k_gauge_force<ACT_WILSON, false, TS, NC, TS, R12>,
+                  k_gauge_force_b<ACT_WILSON, false, TS, NC, TS, R12>, grid_stencil(g), fa);
   }
   const int n = act.n_stout;
   constexpr bool R12 = ops_read12<TS, NC>(1), R12B = ops_read12<TS, NC>(2), R12L = ops_read12<TS, NC>(4);

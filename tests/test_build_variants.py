@@ -1,8 +1,8 @@
 """The compile-time switches of the HIP sources still compile for gfx950 in
 their non-default settings (device code only, no GPU needed): the LDS-tiled
 FP32 Iwasaki top kernel (MAG_TOP_LDS = 12, 20), the FP64 18-real stout
-kernels with full 18-real reads or with two-row reads in the top kernel too
-(MAG_FP64_READ12 = 0, 31), and the local
+kernels with full 18-real reads or with two-row reads in the top kernel and
+the thin force too (MAG_FP64_READ12 = 0, 63), and the local
 Metropolis and checkerboard demon with full reads (MAG_LOCAL_READ12,
 MAG_DEMON_READ12 = 0). About 45 s of hipcc."""
 import os
@@ -19,7 +19,7 @@ CASES = [
     ("inst_f32.hip", "MAG_TOP_LDS=12"),
     ("inst_f32.hip", "MAG_TOP_LDS=20"),
     ("inst_f64.hip", "MAG_FP64_READ12=0"),
-    ("inst_f64.hip", "MAG_FP64_READ12=31"),
+    ("inst_f64.hip", "MAG_FP64_READ12=63"),
     ("local.hip", "MAG_LOCAL_READ12=0"),
     ("demon.hip", "MAG_DEMON_READ12=0"),
 ]
