@@ -44,43 +44,52 @@ constexpr int gforce_waves() { return sizeof(TC) == sizeof(float) ? 3 : 1; }
 #define MAG_GFORCE_OCC __attribute__((amdgpu_waves_per_eu(gforce_waves<TC, CONS>())))
 
 // Blocks are dispatched round-robin over the 8 XCDs (XCD = linear block id
-// mod 8). When an (x0, x1) plane is a whole number of blocks and L3 splits
-// into 8 slabs, XCD k owns x3 in [k L3/8, (k+1) L3/8) and walks it with the
-// direction fastest, then x3, then the (x0, x1) block, then x2: the tiles in
-// flight on one XCD (about 100) then cover whole (x0, x1, x3) slabs at one
-// or two values of x2, so their x3 neighbours are in flight and their x2
-// neighbours were just done. X2_INNER swaps the last two (x2 before the
-// (x0, x1) block): the FP64 top and backward kernels, whose 144-byte links
-// make two x2 layers of whole slabs overflow L2, then find their x2
-// neighbours among the tiles of one block column (FP64 trajectory 2.7 %
-// faster; the forward kernel and FP32 keep the first order). Otherwise XCD k
-// takes tiles [k nb/8, (k+1) nb/8) in lexicographic order.
+// mod 8). When L3 splits into 8 slabs and an x3 slice is a whole number of
+// blocks, XCD k owns x3 in [k L3/8, (k+1) L3/8) and walks it with the
+// direction fastest, then x3, then the blocks of the slice in site order
+// (for a 32^4 lattice: the (x0, x1) block, then x2): the tiles in flight on
+// one XCD (about 100) then cover whole (x0, x1, x3) slabs at one or two
+// values of x2, so their x3 neighbours are in flight and their x2 neighbours
+// were just done. X2_INNER (needs the (x0, x1) plane to be a whole number of
+// blocks) takes x2 before the (x0, x1) block: the FP64 top and backward
+// kernels, whose 144-byte links make two x2 layers of whole slabs overflow
+// L2, then find their x2 neighbours among the tiles of one block column
+// (FP64 trajectory 2.7 % faster; the forward kernel and FP32 keep the first
+// order). Otherwise XCD k takes tiles [k nb/8, (k+1) nb/8) in lexicographic
+// order. nb = blocks in the launch, b = this block; every (mu, sblk) of the
+// 4 V / BLOCK tiles comes out of exactly one b.
 template <bool X2_INNER = false>
-MAG_D void stencil_tile(const Geom& g, int& mu, int& sblk) {
-  const int nb = gridDim.x, b = blockIdx.x;
+MAG_HD void tile_map(const Geom& g, int nb, int b, int& mu, int& sblk) {
   const int plane = g.L[0] * g.L[1];
-  if (nb % 8 == 0 && plane % BLOCK == 0 && g.L[3] % 8 == 0 && nb == 4 * (g.V / BLOCK)) {
+  const int slice = plane * g.L[2];
+  const bool slab = nb % 8 == 0 && g.L[3] % 8 == 0 && nb == 4 * (g.V / BLOCK) &&
+                    (X2_INNER ? plane % BLOCK == 0 : slice % BLOCK == 0);
+  if (slab) {
     const int xcd = b % 8, j = b / 8;
-    const int h3 = g.L[3] / 8, nb01 = plane / BLOCK;
+    const int h3 = g.L[3] / 8;
     mu = j & 3;
     int r = j >> 2;
     const int dx3 = r % h3;
     r /= h3;
-    int b01, x2;
+    int bs;  // block within the x3 slice
     if (X2_INNER) {
-      x2 = r % g.L[2];
-      b01 = r / g.L[2];
+      const int nb01 = plane / BLOCK;
+      const int x2 = r % g.L[2];
+      bs = x2 * nb01 + r / g.L[2];
     } else {
-      b01 = r % nb01;
-      x2 = r / nb01;
+      bs = r;
     }
     const int x3 = xcd * h3 + dx3;
-    sblk = (x3 * g.L[2] + x2) * nb01 + b01;
+    sblk = x3 * (slice / BLOCK) + bs;
     return;
   }
   const int t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
   mu = t & 3;
   sblk = t >> 2;
+}
+template <bool X2_INNER = false>
+MAG_D void stencil_tile(const Geom& g, int& mu, int& sblk) {
+  tile_map<X2_INNER>(g, gridDim.x, blockIdx.x, mu, sblk);
 }
 
 // ======================================================= staples and forces
