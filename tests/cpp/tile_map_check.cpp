@@ -54,7 +54,8 @@ static int check(const int L[4]) {
 int main() {
   const int shapes[][4] = {{4, 4, 4, 4},   {6, 6, 6, 6},    {8, 8, 8, 8},    {12, 12, 12, 12}, {16, 16, 16, 16},
                            {24, 24, 24, 24}, {32, 32, 32, 32}, {16, 16, 4, 8},  {4, 6, 8, 10},    {4, 8, 4, 16},
-                           {8, 4, 8, 8},   {24, 24, 24, 16}, {64, 64, 64, 8}, {4, 4, 4, 8},     {12, 12, 16, 24}};
+                           {8, 4, 8, 8},   {24, 24, 24, 16}, {64, 64, 64, 8}, {4, 4, 4, 8},     {12, 12, 16, 24},
+                           {8, 8, 4, 16}};
   int bad = 0;
   for (const auto& L : shapes) bad |= check<false>(L) | check<true>(L);
   return bad;

@@ -170,11 +170,14 @@ def test_force_fp32_error(ctx, orc, name):
 
 
 # Volumes of the serving stream and non-cubic shapes. The stencil tile order
-# has three paths: XCD slabs (an (x0, x1) plane of whole 256-site blocks and
-# L3 divisible by 8: 16x16x4x8), XCD-interleaved tiles (block count a multiple
-# of 8: 6^4, 4x6x8x10, 12^4, with partial last blocks in the first two) and
-# plain order (4^4). All must give the oracle's force and Hamiltonian.
-@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 6, 6, 6), (4, 6, 8, 10), (12, 12, 12, 12), (16, 16, 4, 8)])
+# has three paths: XCD slabs (an x3 slice of whole 256-site blocks and L3
+# divisible by 8: 16x16x4x8 with whole (x0, x1) planes, 8x8x4x16 with a
+# 64-site plane, where FP64 takes the next path), XCD-interleaved tiles
+# (block count a multiple of 8: 6^4, 4x6x8x10, 12^4, with partial last
+# blocks in the first two) and plain order (4^4). All must give the oracle's
+# force and Hamiltonian.
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 6, 6, 6), (4, 6, 8, 10), (12, 12, 12, 12), (16, 16, 4, 8),
+                               (8, 8, 4, 16)])
 def test_force_volume_sweep(ctx, orc, L):
     a = orc.action(**ACTIONS["iwasaki_stout3"])
     U = orc.haar_links(L, 23)
