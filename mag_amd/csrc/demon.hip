@@ -24,6 +24,9 @@ struct DemonTable {
   // dirs[mu][path][k]: 0..3 forward, 4..7 backward, -1 = end of path
   signed char dirs[4][NPATH][PLEN];
   unsigned char kind[NPATH];  // 0 plaquette, 1 rectangle, 2 chair, 3 twist
+  // paths [kbeg[k], kbeg[k+1]) are of kind k (the same for every direction),
+  // plaquettes of 3 links and the other kinds of 5
+  short kbeg[5];
 };
 
 // Paths of get_staple_dagger (field-hmc.h:226-240), get_rectangular_dagger
@@ -84,6 +87,25 @@ void build_table(DemonTable& t) {
         }
       }
   }
+  int k = 0;
+  for (int q = 0; q < NPATH; ++q)
+    while (k <= t.kind[q]) t.kbeg[k++] = (short)q;
+  while (k <= 4) t.kbeg[k++] = (short)NPATH;
+}
+
+// The checkerboard kernel's assumptions about the table: kinds in order,
+// plaquettes of 3 links, the rest of 5.
+bool table_ok(const DemonTable& t) {
+  for (int q = 0; q < NPATH; ++q) {
+    const int kd = t.kind[q];
+    if (q < t.kbeg[kd] || q >= t.kbeg[kd + 1]) return false;
+    for (int mu = 0; mu < 4; ++mu) {
+      int len = 0;
+      while (len < PLEN && t.dirs[mu][q][len] >= 0) ++len;
+      if (len != (kd == 0 ? 3 : 5)) return false;
+    }
+  }
+  return true;
 }
 
 struct DemonState {
@@ -246,6 +268,9 @@ __global__ __launch_bounds__(BLOCK) void k_staple_table_sum(LinkView<double, 18>
 // and d_MU in {-1, 0, 1}, so two links of one class never meet in a path:
 // the updates of a class commute and each moves its own action change into
 // its own demons. Eight lanes per class member; hit counts in per-member slots.
+// The lanes sum one kind of path at a time (one accumulator; the finished
+// sums wait in LDS), each lane the same paths in the same order as a single
+// loop over all paths, so the sums are bitwise those of that loop.
 struct DemonClassArgs {
   LinkView<double, 18> U;
   Geom g;
@@ -265,19 +290,79 @@ struct DemonClassArgs {
 // neighbouring sites. Surplus lanes of the last wavefront repeat the last
 // member and write nothing. Sweep of a 32^4 lattice on MI355X: 387 ms with
 // 64 lanes per member (every lane's gather its own cache lines), 155 ms with
-// 16, 142 ms with 8, 193 ms with 4 (too few wavefronts per class).
+// 16, 142 ms with 8, 193 ms with 4 (too few wavefronts per class). A class
+// launch at 32^4 is 1,024 one-wave blocks, one wave per SIMD, so the path
+// loads are latency-bound: requesting a path's links together before its
+// products (one round trip per path, not one per link) took the sweep from
+// 107 to 79 ms; 16 lanes at 2 waves per SIMD gave 77.5 ms, and requesting
+// the next path's links during this path's products 93 ms (AGPR copies).
 constexpr int DEMON_LPL = 8;
 // MAG_DEMON_READ12 = 1: the path links are read as rows 0 and 1 with row 2
 // rebuilt (the gathers bound the class kernel; the links stay in SU(3)).
 #ifndef MAG_DEMON_READ12
 #define MAG_DEMON_READ12 1
 #endif
+// The paths of one kind, of LEN links each, that lane sub of a member sums
+// (q = sub mod LPL, in increasing q, as one loop over all paths would give
+// it): the links of a path are all requested before its products (sdir is
+// the direction's path table in LDS).
+template <int LPL, int LEN>
+MAG_D void demon_kind_sum(const DemonClassArgs& A, const signed char* sdir, int qa, int qb, int sub, const int x[4],
+                          M3<double>& X) {
+  X = zero3<double>();
+#pragma unroll 1
+  for (int q = qa + ((sub - qa) % LPL + LPL) % LPL; q < qb; q += LPL) {
+    int y[4] = {x[0], x[1], x[2], x[3]};
+    M3<double> l[LEN];
+    bool back[LEN];
+#pragma unroll
+    for (int j = 0; j < LEN; ++j) {
+      const int d = sdir[q * PLEN + j];
+      const int k = d & 3;
+      back[j] = d >= 4;
+      if (back[j]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i == k) y[i] = (y[i] == 0) ? A.g.L[i] - 1 : y[i] - 1;
+      }
+      const int sy = y[0] + A.g.S[1] * y[1] + A.g.S[2] * y[2] + A.g.S[3] * y[3];
+      l[j] = MAG_DEMON_READ12 ? A.U.template load_r2<double>(k, sy) : A.U.template load<double>(k, sy);
+      if (!back[j]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i == k) y[i] = (y[i] == A.g.L[i] - 1) ? 0 : y[i] + 1;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the path's loads ahead of its products
+    M3<double> m;
+#pragma unroll
+    for (int j = 0; j < LEN; ++j) {
+      if (MAG_DEMON_READ12) reconstruct_row2(l[j]);
+      if (j == 0) m = back[0] ? dag(l[0]) : l[0];
+      else m = back[j] ? mul_nd(m, l[j]) : mul(m, l[j]);
+    }
+    add_to(X, m);
+  }
+  // butterfly over the member's lanes: each ends with the full sum
+#pragma unroll
+  for (int off = LPL / 2; off >= 1; off >>= 1)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      X.e[e].r += __shfl_xor(X.e[e].r, off, 64);
+      X.e[e].i += __shfl_xor(X.e[e].i, off, 64);
+    }
+}
+
 template <int MU, int LPL>
 __global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
   constexpr int MPW = 64 / LPL;
+  __shared__ signed char sdir[NPATH * PLEN];
+  __shared__ double sum[MPW][4][18];  // a member's four staple sums, written by its lane 0
   const int lane = threadIdx.x;
-  const int sub = lane % LPL;
-  const int t0 = blockIdx.x * MPW + lane / LPL;
+  for (int i = lane; i < NPATH * PLEN; i += 64) sdir[i] = (&A.tab->dirs[MU][0][0])[i];
+  __syncthreads();
+  const int sub = lane % LPL, mem = lane / LPL;
+  const int t0 = blockIdx.x * MPW + mem;
   const bool live = t0 < A.nthr;
   const int t = live ? t0 : A.nthr - 1;
   int x[4], r = t, c = A.cls;
@@ -294,53 +379,27 @@ __global__ __launch_bounds__(64) void k_demon_class(DemonClassArgs A) {
     }
   }
   const int s = x[0] + A.g.S[1] * x[1] + A.g.S[2] * x[2] + A.g.S[3] * x[3];
-  // this lane's share of the four staple sums (same paths and kinds as the
-  // sequential demon)
-  M3<double> X[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) X[k] = zero3<double>();
+  // the four staple sums (same paths and kinds as the sequential demon), one
+  // kind at a time
 #pragma unroll 1
-  for (int q = sub; q < NPATH; q += LPL) {
-    int y[4] = {x[0], x[1], x[2], x[3]};
-    M3<double> m;
-    bool fresh = true;
-#pragma unroll 1
-    for (int j = 0; j < PLEN; ++j) {
-      const int d = A.tab->dirs[MU][q][j];
-      if (d < 0) break;
-      const int k = d & 3, n = d < 4 ? 1 : -1;
-      if (n < 0) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (i == k) y[i] = (y[i] == 0) ? A.g.L[i] - 1 : y[i] - 1;
-      }
-      const int sy = y[0] + A.g.S[1] * y[1] + A.g.S[2] * y[2] + A.g.S[3] * y[3];
-      const M3<double> l = A.U.template load_su3<double, MAG_DEMON_READ12 != 0>(k, sy);
-      if (fresh) m = (n > 0) ? l : dag(l);
-      else m = (n > 0) ? mul(m, l) : mul_nd(m, l);
-      fresh = false;
-      if (n > 0) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (i == k) y[i] = (y[i] == A.g.L[i] - 1) ? 0 : y[i] + 1;
-      }
-    }
-    const int kd = A.tab->kind[q];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (k == kd) add_to(X[k], m);
-  }
-  // butterfly over the member's lanes: each ends with the full sums
-#pragma unroll
-  for (int off = LPL / 2; off >= 1; off >>= 1)
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
+  for (int kd = 0; kd < 4; ++kd) {
+    M3<double> X;
+    if (kd == 0) demon_kind_sum<LPL, 3>(A, sdir, A.tab->kbeg[0], A.tab->kbeg[1], sub, x, X);
+    else demon_kind_sum<LPL, 5>(A, sdir, A.tab->kbeg[kd], A.tab->kbeg[kd + 1], sub, x, X);
+    if (sub == 0) {
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
-        X[k].e[e].r += __shfl_xor(X[k].e[e].r, off, 64);
-        X[k].e[e].i += __shfl_xor(X[k].e[e].i, off, 64);
+        sum[mem][kd][2 * e] = X.e[e].r;
+        sum[mem][kd][2 * e + 1] = X.e[e].i;
       }
+    }
+  }
   if (sub != 0 || !live) return;
+  M3<double> X[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) X[k].e[e] = {sum[mem][k][2 * e], sum[mem][k][2 * e + 1]};
   // plaquette and rectangle sums are used daggered, chair and twist as they are
   X[0] = dag(X[0]);
   X[1] = dag(X[1]);
@@ -456,6 +515,10 @@ extern "C" int mag_demon_checkerboard(mag_field U, mag_field Ep, mag_field Er, m
   if ((r = fill_dev(c, cnt, 0., 2 * nthr))) return r;
   DemonTable* ht = new DemonTable();
   build_table(*ht);
+  if (!table_ok(*ht)) {
+    delete ht;
+    return fail(c, MAG_ERR_STATE, "demon path table does not have the layout the class kernel assumes");
+  }
   DemonTable* dt = nullptr;
   r = dev_alloc(c, (void**)&dt, sizeof(DemonTable));
   if (!r && hipMemcpyAsync(dt, ht, sizeof(DemonTable), hipMemcpyHostToDevice, c->stream) != hipSuccess)
