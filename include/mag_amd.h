@@ -103,6 +103,9 @@ int mag_ctx_memory(mag_ctx ctx, size_t* bytes);
 /* Time of the kernels of the last timed region (see mag_ctx_set_timing):
  * per-kernel-family event time, milliseconds. */
 int mag_ctx_set_timing(mag_ctx ctx, int on);
+/* While timing is on, time only the launches of one kernel family (NULL or
+ * "" = every family): fewer event records inside a timed region. */
+int mag_ctx_set_timing_family(mag_ctx ctx, const char* family);
 int mag_ctx_kernel_time(mag_ctx ctx, const char* family, double* ms, long* launches);
 int mag_ctx_reset_timing(mag_ctx ctx);
 

@@ -86,6 +86,10 @@ class Context:
     def set_timing(self, on):
         _check(_lib.lib().mag_ctx_set_timing(self._h, 1 if on else 0), self._h)
 
+    def set_timing_family(self, family=None):
+        """Time only this kernel family while timing is on (None: every family)."""
+        _check(_lib.lib().mag_ctx_set_timing_family(self._h, family.encode() if family else None), self._h)
+
     def reset_timing(self):
         _check(_lib.lib().mag_ctx_reset_timing(self._h), self._h)
 

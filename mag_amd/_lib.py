@@ -73,6 +73,7 @@ SIGNATURES = {
     "mag_ctx_synchronize": [_P],
     "mag_ctx_memory": [_P, C.POINTER(C.c_size_t)],
     "mag_ctx_set_timing": [_P, C.c_int],
+    "mag_ctx_set_timing_family": [_P, C.c_char_p],
     "mag_ctx_kernel_time": [_P, C.c_char_p, _D, _L],
     "mag_ctx_reset_timing": [_P],
     "mag_field_create": [_P, _I, C.c_int, C.c_int, C.POINTER(_P)],

@@ -186,6 +186,12 @@ int mag_ctx_set_timing(mag_ctx c, int on) {
   return MAG_OK;
 }
 
+int mag_ctx_set_timing_family(mag_ctx c, const char* family) {
+  if (!c) return MAG_ERR_ARG;
+  c->timing_family = family ? family : "";
+  return MAG_OK;
+}
+
 int mag_ctx_reset_timing(mag_ctx c) {
   if (!c) return MAG_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));

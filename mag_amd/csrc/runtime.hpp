@@ -71,6 +71,7 @@ struct mag_ctx_s {
   size_t bytes = 0;
   std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<mag::Workspace>> ws;
   bool timing = false;
+  std::string timing_family;  // when not empty, only this kernel family is timed
   std::map<std::string, mag::TimerRec> timers;
   std::vector<hipEvent_t> event_pool;
   mag::Recorder* rec = nullptr;  // set while a serving batch records its launches
@@ -121,20 +122,24 @@ inline hipEvent_t take_event(mag_ctx c) {
 }
 
 // Records HIP events around one kernel launch on the context's stream when
-// timing is on; elapsed times are collected lazily by collect_timers.
+// timing is on (for every family, or only the one set by
+// mag_ctx_set_timing_family); elapsed times are collected lazily by
+// collect_timers.
 struct KTimer {
   mag_ctx c;
   const char* name;
+  bool on;
   hipEvent_t a = nullptr, b = nullptr;
-  KTimer(mag_ctx c_, const char* n) : c(c_), name(n) {
-    if (c->timing) {
+  KTimer(mag_ctx c_, const char* n)
+      : c(c_), name(n), on(c_->timing && (c_->timing_family.empty() || c_->timing_family == n)) {
+    if (on) {
       a = take_event(c);
       b = take_event(c);
       (void)hipEventRecord(a, c->stream);
     }
   }
   ~KTimer() {
-    if (c->timing) {
+    if (on) {
       (void)hipEventRecord(b, c->stream);
       c->timers[name].pending.emplace_back(a, b);
     }
