@@ -248,27 +248,31 @@ def timed_chain(M, ctx, L, arg, seed, precision, start, warmup, steps, timing, w
     trajectories, then `steps` trajectories timed together between two device
     synchronisations (with `wloop`, one r = (4,0,0), T = 4 Wilson loop after
     each trajectory, inside the timed region). With `dominant_only` (a dict)
-    and warm-up trajectories, the warm-up is kernel-timed family by family
-    (per trajectory into `dominant_only`) and the timed region times only the
-    family that took longest. Returns (seconds, warm-up results, timed
+    and at least two warm-up trajectories, the warm-up after its first
+    trajectory is kernel-timed family by family (per trajectory into
+    `dominant_only`) and the timed region times only the family that took
+    longest. Returns (seconds, warm-up results, timed
     results, chain, Wilson loops)."""
     ch = M.Chain(ctx, (L,) * 4, arg, seed=seed, precision=precision, accept_mode=2, demon_emax=-0.005,
                  recon12=recon12)
     ch.links().copy_from(start.links())
     ch.reset_demon()
     ch.demon_state()  # window centre from the starting links, outside the timed region
-    if timing and dominant_only is not None and warmup:
-        # the per-kernel breakdown from the warm-up trajectories; inside the
+    if timing and dominant_only is not None and warmup >= 2:
+        # the per-kernel breakdown from the warm-up trajectories after the
+        # first (which also evaluates the starting Hamiltonian); inside the
         # timed region only the dominant kernel carries events
+        pre = ch.run(1)
+        ctx.synchronize()
         ctx.reset_timing()
         ctx.set_timing(True)
-        pre = ch.run(warmup)
+        pre += ch.run(warmup - 1)
         ctx.synchronize()
         warm = {}
         for f in list(KERNEL_UNITS) + ["stout_fwd_h", "action", "kinetic", "reduce", "mom_refresh", "reunitarize"]:
             ms, nl = ctx.kernel_time(f)
             if nl:
-                warm[f] = (ms / warmup, nl / warmup)
+                warm[f] = (ms / (warmup - 1), nl / (warmup - 1))
         ctx.set_timing(False)
         dominant_only.update(warm)
         dom = max((f for f in warm if f in KERNEL_UNITS), key=lambda f: warm[f][0])
@@ -628,8 +632,8 @@ def main():
            "thermalisation": therm_info,
            "dh": [round(r.dh, 5) for r in res],
            "kernel_ms_per_traj": {f: round(v[0] / a.steps, 3) for f, v in fams.items()},
-           "kernel_ms_source": (f"{dom}: HIP events over the timed region; the other kernels: HIP events over the "
-                                f"{a.warmup} warm-up trajectories" if warm and len(timed_families) == 1 else
+           "kernel_ms_source": (f"{dom}: HIP events over the timed region; the other kernels: HIP events over "
+                                f"warm-up trajectories 2-{a.warmup}" if warm and len(timed_families) == 1 else
                                 "HIP events over the timed region"),
            "roofline": roof}
     if loops:
