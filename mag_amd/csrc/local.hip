@@ -347,13 +347,12 @@ int mag_local_metropolis(mag_field U, double beta, int mag, int num_iter, int nh
     }
   }
   if (acc_rej) {
-    std::vector<double> h((size_t)4 * nthr);
-    HIPCHK(c, hipMemcpyAsync(h.data(), cnt, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double s[4] = {0., 0., 0., 0.};
-    for (int t = 0; t < nthr; ++t)
-      for (int k = 0; k < 4; ++k) s[k] += h[(size_t)t * 4 + k];
-    for (int k = 0; k < 4; ++k) acc_rej[k] = (long)s[k];
+    // the per-thread counts summed on the device (whole numbers: exact in
+    // any order), four scalars read back
+    for (int k = 0; k < 4; ++k)
+      if ((r = final_reduce(c, cnt + k, nthr, 4, 0, k))) return r;
+    if ((r = fetch_scalars(c, 4))) return r;
+    for (int k = 0; k < 4; ++k) acc_rej[k] = (long)c->h_scalar[k];
   }
   return MAG_OK;
 }
